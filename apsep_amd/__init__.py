@@ -1,0 +1,173 @@
+"""apsep_amd -- MI355X-native APSEP / ANSV-left engine (host bindings).
+
+The product is the gfx950 HIP library ``apsep_amd/libapsep.so`` and its C ABI
+(``include/apsep/apsep.h``); C++ callers use the reference-named shims in
+``include/apsep/apsep.hpp``.  This module binds the same C ABI with ctypes
+for Python callers, tests and the benchmark.  PyTorch is used only for
+device memory and streams: every solve runs the HIP kernels, and the module
+raises if the library is missing -- there is no CPU fallback.
+
+Entry points mirror the reference's host API (src/apsep.cuh:2503-2605,
+3001-3076):
+
+    Scratch(algo, n_cap, incl)      ~ allocWSTLScratch / allocSPTScratch
+    Scratch.close()                 ~ freeWSTLScratch / freeSPTScratch
+    Scratch.run(d_in, d_out, n)     ~ runWSTL / runSPT (async on a stream)
+    launch(algo, d_in, d_out, n)    ~ launchWSTL / launchSPT (sync)
+    solve(x)                        convenience: allocate the output, solve
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libapsep.so")
+
+OK, ERR_INVALID_ARG, ERR_ALIGN, ERR_CAPACITY, ERR_ALLOC, ERR_LAUNCH, \
+    ERR_DEVICE, ERR_TOO_LARGE, ERR_UNSUPPORTED, ERR_BUDGET = range(10)
+ALGO_WSTL, ALGO_SPT, ALGO_LOOKBACK, ALGO_AUTO = range(4)
+ALGOS = {"wstl": ALGO_WSTL, "spt": ALGO_SPT, "lookback": ALGO_LOOKBACK,
+         "auto": ALGO_AUTO}
+
+# every symbol include/apsep/apsep.h declares
+EXPORTS = [
+    "apsep_status_string", "apsep_version", "apsep_scratch_create",
+    "apsep_scratch_destroy", "apsep_scratch_bytes", "apsep_run",
+    "apsep_launch", "apsep_run_mode", "apsep_scratch_query",
+]
+
+_lib = None
+
+
+class ScratchInfo(ctypes.Structure):
+    """apsep_scratch_info (include/apsep/apsep.h)."""
+    _fields_ = [("n_cap", ctypes.c_int64), ("bytes", ctypes.c_int64),
+                ("algo", ctypes.c_int32), ("incl", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("tile_elems", ctypes.c_int32),
+                ("tiles", ctypes.c_int32), ("levels", ctypes.c_int32),
+                ("persistent_blocks", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class ApsepError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        msg = lib().apsep_status_string(status).decode()
+        super().__init__("%s%s (status %d)" % (what + ": " if what else "", msg, status))
+
+
+def lib() -> ctypes.CDLL:
+    """Load libapsep.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("apsep_amd/libapsep.so is not built; run "
+                              "`python -m apsep_amd.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, i32, ci = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int
+        L.apsep_status_string.argtypes = [ci]
+        L.apsep_status_string.restype = ctypes.c_char_p
+        L.apsep_version.restype = ctypes.c_char_p
+        L.apsep_scratch_create.argtypes = [ci, i64, ci, ctypes.POINTER(vp)]
+        L.apsep_scratch_destroy.argtypes = [vp]
+        L.apsep_scratch_bytes.argtypes = [ci, i64, ci, ctypes.POINTER(i64)]
+        L.apsep_run.argtypes = [vp, vp, vp, i64, vp]
+        L.apsep_launch.argtypes = [ci, vp, vp, i64, ci]
+        L.apsep_run_mode.argtypes = [vp, vp, vp, i64, ci, vp]
+        L.apsep_scratch_query.argtypes = [vp, ctypes.POINTER(ScratchInfo)]
+        for f in EXPORTS:
+            if f != "apsep_status_string" and f != "apsep_version":
+                getattr(L, f).restype = ci
+        _lib = L
+    return _lib
+
+
+def _check(st: int, what: str = ""):
+    if st != OK:
+        raise ApsepError(st, what)
+
+
+def _algo(a) -> int:
+    if isinstance(a, str):
+        return ALGOS[a.lower()]
+    return int(a)
+
+
+def scratch_bytes(algo, n_cap: int, incl: bool = False) -> int:
+    b = ctypes.c_int64(0)
+    _check(lib().apsep_scratch_bytes(_algo(algo), n_cap, int(incl), ctypes.byref(b)),
+           "apsep_scratch_bytes")
+    return b.value
+
+
+class Scratch:
+    """Device scratch for solves of up to ``n_cap`` elements."""
+
+    def __init__(self, algo="spt", n_cap: int = 0, incl: bool = False):
+        self._h = ctypes.c_void_p(None)
+        self.n_cap = n_cap
+        self.incl = bool(incl)
+        _check(lib().apsep_scratch_create(_algo(algo), n_cap, int(incl),
+                                          ctypes.byref(self._h)),
+               "apsep_scratch_create")
+        self.info = ScratchInfo()
+        _check(lib().apsep_scratch_query(self._h, ctypes.byref(self.info)))
+        self.algo = self.info.algo
+
+    def run_ptr(self, d_in: int, d_out: int, n: int, stream: int = 0,
+                incl: bool | None = None):
+        """Raw-pointer solve (async on ``stream``)."""
+        if incl is None:
+            _check(lib().apsep_run(self._h, d_in, d_out, n, stream), "apsep_run")
+        else:
+            _check(lib().apsep_run_mode(self._h, d_in, d_out, n, int(incl), stream),
+                   "apsep_run_mode")
+
+    def run(self, x, out, n: int | None = None, stream=None, incl: bool | None = None):
+        """Solve ``x`` (int32 CUDA tensor) into ``out`` on ``stream`` (default:
+        torch's current stream)."""
+        import torch
+        if n is None:
+            n = x.numel()
+        if x.dtype != torch.int32 or out.dtype != torch.int32:
+            raise TypeError("apsep: int32 tensors required")
+        if not (x.is_cuda and out.is_cuda and x.is_contiguous() and out.is_contiguous()):
+            raise ValueError("apsep: contiguous device tensors required")
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        self.run_ptr(x.data_ptr(), out.data_ptr(), n, stream.cuda_stream, incl)
+        return out
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().apsep_scratch_destroy(self._h)
+        self._h = ctypes.c_void_p(None)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def launch(algo, d_in: int, d_out: int, n: int, incl: bool = False):
+    """Synchronous solve with internal scratch (launchWSTL / launchSPT)."""
+    _check(lib().apsep_launch(_algo(algo), d_in, d_out, n, int(incl)), "apsep_launch")
+
+
+def solve(x, algo="spt", incl: bool = False, out=None):
+    """Convenience: allocate ``out`` (int32, like ``x``) and solve on torch's
+    current stream."""
+    import torch
+    if out is None:
+        out = torch.empty_like(x, dtype=torch.int32)
+    with Scratch(algo, x.numel(), incl) as s:
+        s.run(x, out)
+        torch.cuda.current_stream(x.device).synchronize()
+    return out

@@ -1,0 +1,164 @@
+// apsep_api.hip -- the extern "C" boundary (include/apsep/apsep.h).
+//
+// Argument checks run before any HIP call, so every error path is testable
+// on a machine without a GPU.  Nothing here prints or exits.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include <new>
+
+#include "apsep_device.hpp"
+#include "apsep_internal.hpp"
+#include "../../include/apsep/apsep.h"
+
+using namespace apsep;
+
+extern "C" {
+
+const char* apsep_status_string(apsep_status s) {
+    switch (s) {
+    case APSEP_OK: return "ok";
+    case APSEP_ERR_INVALID_ARG: return "invalid argument";
+    case APSEP_ERR_ALIGN: return "device pointer not 16-byte aligned";
+    case APSEP_ERR_CAPACITY: return "n exceeds the scratch capacity";
+    case APSEP_ERR_ALLOC: return "device allocation failed";
+    case APSEP_ERR_LAUNCH: return "kernel launch or runtime call failed";
+    case APSEP_ERR_DEVICE: return "no usable HIP device (or not the scratch's device)";
+    case APSEP_ERR_TOO_LARGE: return "n exceeds INT32_MAX";
+    case APSEP_ERR_UNSUPPORTED: return "operation not supported on this device";
+    case APSEP_ERR_BUDGET: return "request exceeds the memory budget";
+    }
+    return "unknown status";
+}
+
+const char* apsep_version(void) { return "0.1.0"; }
+
+static bool valid_algo(apsep_algo a) {
+    return a == APSEP_ALGO_WSTL || a == APSEP_ALGO_SPT || a == APSEP_ALGO_LOOKBACK ||
+           a == APSEP_ALGO_AUTO;
+}
+
+static apsep_algo resolve_algo(apsep_algo a) {
+    if (a == APSEP_ALGO_AUTO) return APSEP_ALGO_SPT;
+    if (a == APSEP_ALGO_LOOKBACK) return APSEP_ALGO_SPT;  // not built yet
+    return a;
+}
+
+apsep_status apsep_scratch_bytes(apsep_algo algo, int64_t n_cap, int incl,
+                                 int64_t* bytes) {
+    if (!bytes || !valid_algo(algo) || n_cap < 0 || (incl != 0 && incl != 1))
+        return APSEP_ERR_INVALID_ARG;
+    if (n_cap > INT32_MAX) return APSEP_ERR_TOO_LARGE;
+    *bytes = n_cap > 0 ? single_scratch_bytes(n_cap) : 0;
+    return APSEP_OK;
+}
+
+apsep_status apsep_scratch_create(apsep_algo algo, int64_t n_cap, int incl,
+                                  apsep_scratch** out) {
+    if (!out) return APSEP_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!valid_algo(algo) || n_cap < 0 || (incl != 0 && incl != 1))
+        return APSEP_ERR_INVALID_ARG;
+    if (n_cap > INT32_MAX) return APSEP_ERR_TOO_LARGE;
+    int dev = -1, ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 ||
+        hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return APSEP_ERR_DEVICE;
+    }
+    apsep_scratch* s = new (std::nothrow) apsep_scratch;
+    if (!s) return APSEP_ERR_ALLOC;
+    memset(s, 0, sizeof *s);
+    s->algo = resolve_algo(algo);
+    s->incl = incl;
+    s->n_cap = n_cap;
+    s->device = dev;
+    if (hipDeviceGetAttribute(&s->num_cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        delete s;
+        return APSEP_ERR_DEVICE;
+    }
+    apsep_status st = single_scratch_init(s);
+    if (st != APSEP_OK) {
+        apsep_scratch_destroy(s);
+        return st;
+    }
+    *out = s;
+    return APSEP_OK;
+}
+
+apsep_status apsep_scratch_destroy(apsep_scratch* s) {
+    if (!s) return APSEP_OK;
+    if (s->base) (void)hipFree(s->base);
+    delete s;
+    return APSEP_OK;
+}
+
+apsep_status apsep_scratch_query(const apsep_scratch* s, apsep_scratch_info* info) {
+    if (!s || !info) return APSEP_ERR_INVALID_ARG;
+    memset(info, 0, sizeof *info);
+    int sz[MAX_LEVELS];
+    info->n_cap = s->n_cap;
+    info->bytes = s->bytes;
+    info->algo = (int32_t)s->algo;
+    info->incl = s->incl;
+    info->device = s->device;
+    info->tile_elems = TILE;
+    info->tiles = s->nt_cap;
+    info->levels = s->nt_cap > 0 ? level_sizes(s->nt_cap, sz) : 0;
+    info->persistent_blocks = s->coop_grid;
+    return APSEP_OK;
+}
+
+static apsep_status check_io(const int32_t* in, int32_t* out, int64_t n) {
+    if (!in || !out) return APSEP_ERR_INVALID_ARG;
+    if (n > INT32_MAX) return APSEP_ERR_TOO_LARGE;
+    if (((uintptr_t)in & 15) || ((uintptr_t)out & 15)) return APSEP_ERR_ALIGN;
+    const uintptr_t a0 = (uintptr_t)in, a1 = a0 + (uintptr_t)n * 4;
+    const uintptr_t b0 = (uintptr_t)out, b1 = b0 + (uintptr_t)n * 4;
+    if (a0 < b1 && b0 < a1) return APSEP_ERR_INVALID_ARG;  // overlap
+    return APSEP_OK;
+}
+
+apsep_status apsep_run(apsep_scratch* s, const int32_t* d_in, int32_t* d_out,
+                       int64_t n, void* stream) {
+    if (!s) return APSEP_ERR_INVALID_ARG;
+    return apsep_run_mode(s, d_in, d_out, n, s->incl, stream);
+}
+
+apsep_status apsep_run_mode(apsep_scratch* s, const int32_t* d_in, int32_t* d_out,
+                            int64_t n, int incl, void* stream) {
+    if (!s || (incl != 0 && incl != 1)) return APSEP_ERR_INVALID_ARG;
+    if (n <= 0) return APSEP_OK;
+    if (n > INT32_MAX) return APSEP_ERR_TOO_LARGE;
+    if (n > s->n_cap) return APSEP_ERR_CAPACITY;
+    apsep_status st = check_io(d_in, d_out, n);
+    if (st != APSEP_OK) return st;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != s->device) {
+        (void)hipGetLastError();
+        return APSEP_ERR_DEVICE;
+    }
+    return single_run(s, d_in, d_out, (int)n, incl, (hipStream_t)stream);
+}
+
+apsep_status apsep_launch(apsep_algo algo, const int32_t* d_in, int32_t* d_out,
+                          int64_t n, int incl) {
+    if (!valid_algo(algo) || (incl != 0 && incl != 1)) return APSEP_ERR_INVALID_ARG;
+    if (n <= 0) return APSEP_OK;
+    apsep_status st = check_io(d_in, d_out, n);
+    if (st != APSEP_OK) return st;
+    apsep_scratch* s = nullptr;
+    st = apsep_scratch_create(algo, n, incl, &s);
+    if (st != APSEP_OK) return st;
+    st = apsep_run(s, d_in, d_out, n, nullptr);
+    if (st == APSEP_OK && hipDeviceSynchronize() != hipSuccess) {
+        (void)hipGetLastError();
+        st = APSEP_ERR_LAUNCH;
+    }
+    apsep_scratch_destroy(s);
+    return st;
+}
+
+}  // extern "C"

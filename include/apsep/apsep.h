@@ -1,0 +1,117 @@
+/*
+ * apsep.h -- C ABI of the MI355X (gfx950) APSEP / ANSV-left engine.
+ *
+ * Problem: for every d_in[i] (int32), d_out[i] = the largest j < i with
+ * d_in[j] < d_in[i] (strict, incl = 0) or d_in[j] <= d_in[i] (incl = 1),
+ * else -1.  Output is int32.
+ *
+ * Every entry point is extern "C" with plain pointers and sizes: no C++ or
+ * torch types cross this boundary.  Device pointers come from hipMalloc (or
+ * any allocator giving 16-byte aligned device memory); streams are
+ * hipStream_t passed as void* (NULL = the default stream).
+ *
+ * Errors are returned, never raised: the library does not print or exit.
+ * The reference exits through gpuAssert (src/apsep.cuh:39-47); the C++ shims
+ * in apsep.hpp restore that behaviour for drop-in callers.
+ *
+ * Threading: one apsep_scratch per concurrent caller; a scratch is bound to
+ * the device that was current when it was created.
+ */
+#ifndef APSEP_APSEP_H
+#define APSEP_APSEP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum apsep_status {
+    APSEP_OK = 0,
+    APSEP_ERR_INVALID_ARG = 1,  /* null pointer, bad enum, negative size   */
+    APSEP_ERR_ALIGN = 2,        /* d_in / d_out not 16-byte aligned         */
+    APSEP_ERR_CAPACITY = 3,     /* n larger than the scratch was sized for  */
+    APSEP_ERR_ALLOC = 4,        /* device allocation failed                 */
+    APSEP_ERR_LAUNCH = 5,       /* kernel launch / runtime call failed      */
+    APSEP_ERR_DEVICE = 6,       /* no usable HIP device, or wrong device    */
+    APSEP_ERR_TOO_LARGE = 7,    /* n > INT32_MAX (int32 indices)            */
+    APSEP_ERR_UNSUPPORTED = 8,  /* e.g. cooperative launch not supported    */
+    APSEP_ERR_BUDGET = 9        /* serving: job exceeds the memory budget   */
+} apsep_status;
+
+/* Solver variants.  WSTL = multi-launch pipeline (tile pass, 64-ary tree
+ * build, fix-up pass); SPT = one cooperative persistent launch with grid
+ * barriers; LOOKBACK = one non-cooperative launch with decoupled look-back;
+ * AUTO = the fastest measured for the size. */
+typedef enum apsep_algo {
+    APSEP_ALGO_WSTL = 0,
+    APSEP_ALGO_SPT = 1,
+    APSEP_ALGO_LOOKBACK = 2,
+    APSEP_ALGO_AUTO = 3
+} apsep_algo;
+
+typedef struct apsep_scratch apsep_scratch;
+
+/* Human-readable text for a status code (static storage). */
+const char* apsep_status_string(apsep_status s);
+
+/* Library version as "major.minor.patch". */
+const char* apsep_version(void);
+
+/* Replaces allocWSTLScratch (src/apsep.cuh:2520-2537) and allocSPTScratch
+ * (src/apsep.cuh:3013-3040): device scratch sized for up to n_cap elements,
+ * bound to the current device.  n_cap may be 0 (then only n = 0 runs). */
+apsep_status apsep_scratch_create(apsep_algo algo, int64_t n_cap, int incl,
+                                  apsep_scratch** out);
+
+/* Replaces freeWSTLScratch (src/apsep.cuh:2539-2546) and freeSPTScratch
+ * (src/apsep.cuh:3042-3050).  NULL is accepted. */
+apsep_status apsep_scratch_destroy(apsep_scratch* s);
+
+/* Bytes of device memory a scratch for n_cap elements holds. */
+apsep_status apsep_scratch_bytes(apsep_algo algo, int64_t n_cap, int incl,
+                                 int64_t* bytes);
+
+/* Replaces runWSTL (src/apsep.cuh:2548-2596) and runSPT
+ * (src/apsep.cuh:3052-3067): asynchronous solve on `stream`.
+ * n <= 0 is a no-op returning APSEP_OK (as the reference returns early);
+ * n > capacity -> APSEP_ERR_CAPACITY.  d_in and d_out: n int32 each,
+ * 16-byte aligned, not overlapping.  d_out needs no initialisation. */
+apsep_status apsep_run(apsep_scratch* s, const int32_t* d_in, int32_t* d_out,
+                       int64_t n, void* stream);
+
+/* Replaces launchWSTL (src/apsep.cuh:2598-2605) and launchSPT
+ * (src/apsep.cuh:3069-3076): allocate scratch, run on the default stream,
+ * synchronise, free. */
+apsep_status apsep_launch(apsep_algo algo, const int32_t* d_in, int32_t* d_out,
+                          int64_t n, int incl);
+
+/* Same as apsep_run with the predicate chosen per call (incl = 0: strict
+ * "<", 1: "<="), whatever the scratch was created with.  Replaces the INCL
+ * template argument of runSPT (src/apsep.cuh:3051-3052), which the
+ * reference also picks per call on an INCL-agnostic scratch. */
+apsep_status apsep_run_mode(apsep_scratch* s, const int32_t* d_in,
+                            int32_t* d_out, int64_t n, int incl, void* stream);
+
+/* Shape of a scratch: the counterpart of the fields of WSTLScratch /
+ * SPTScratch (src/apsep.cuh:2509-2518, 3001-3011). */
+typedef struct apsep_scratch_info {
+    int64_t n_cap;             /* capacity in elements                      */
+    int64_t bytes;             /* device bytes held                         */
+    int32_t algo;              /* resolved apsep_algo (never AUTO)          */
+    int32_t incl;              /* default predicate                         */
+    int32_t device;            /* HIP device the scratch lives on           */
+    int32_t tile_elems;        /* elements per tile (workgroup)             */
+    int32_t tiles;             /* tiles at capacity (num_blocks)            */
+    int32_t levels;            /* levels of the 64-ary tile-min hierarchy   */
+    int32_t persistent_blocks; /* SPT cooperative grid (num_phys), else 0   */
+    int32_t reserved;
+} apsep_scratch_info;
+
+apsep_status apsep_scratch_query(const apsep_scratch* s, apsep_scratch_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APSEP_APSEP_H */

@@ -1,0 +1,203 @@
+"""Bit-exact parity of the gfx950 solvers against the CPU oracle.
+
+Mirrors the reference's own tests:
+  * structured known-answer cases (src/main.cu:181-222, fixture in
+    tests/golden/known_answers.json);
+  * seeded stress, std::mt19937 seeds 42/43/100/101 (src/main.cu:144-175,
+    246-252);
+  * the <, <= matrix of src/bench_spt_le.cu:93-125 (glibc rand() inputs at
+    n in {2^25, 1000003, 513, 512, 511, 1}, output poisoned with 0xAB);
+plus the benchmark configurations, tile/level boundary sizes, adversarial
+inputs that reach the overflow and deep-search paths, and determinism.
+Every call goes through the C ABI (libapsep.so via apsep_amd).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+ALGOS = ["wstl", "spt"]
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+POISON = np.int32(np.uint32(0xABABABAB).view(np.int32))
+
+
+def _log(*a):
+    print("[parity]", *a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def solve_on(scratch, torch, host, incl=None, x_dev=None, out_dev=None):
+    n = host.shape[0]
+    x = torch.from_numpy(np.ascontiguousarray(host, dtype=np.int32)).cuda() \
+        if x_dev is None else x_dev
+    out = torch.full((max(n, 1),), int(POISON), dtype=torch.int32, device="cuda") \
+        if out_dev is None else out_dev
+    scratch.run(x, out, n=n, incl=incl)
+    torch.cuda.synchronize()
+    return out[:n].cpu().numpy()
+
+
+def check(got, a, incl, what):
+    exp = oracle.apsep(a, incl)
+    if not np.array_equal(got, exp):
+        bad = np.nonzero(got != exp)[0]
+        i = int(bad[0])
+        raise AssertionError("%s: %d mismatches, first i=%d val=%d got=%d exp=%d"
+                             % (what, bad.size, i, a[i], got[i], exp[i]))
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+def test_known_answers(torch_mod, algo):
+    import apsep_amd as A
+    with open(os.path.join(GOLDEN, "known_answers.json")) as f:
+        cases = json.load(f)["cases"]
+    with A.Scratch(algo, 4096) as s:
+        for c in cases:
+            a = np.array(c["input"], dtype=np.int32)
+            for incl, key in ((False, "strict"), (True, "incl")):
+                _log(algo, c["label"], key)
+                got = solve_on(s, torch_mod, a, incl)
+                assert got.tolist() == c[key], (algo, c["label"], key)
+
+
+@pytest.mark.parametrize("algo,seed,trials,maxn", [
+    ("wstl", 42, 500, 4096), ("spt", 43, 500, 4096),
+    ("wstl", 100, 200, 131072), ("spt", 101, 200, 131072)])
+def test_reference_stress(torch_mod, algo, seed, trials, maxn):
+    import apsep_amd as A
+    torch = torch_mod
+    xs = torch.empty(maxn, dtype=torch.int32, device="cuda")
+    outs = torch.empty(maxn, dtype=torch.int32, device="cuda")
+    _log(algo, "stress seed", seed, trials, maxn)
+    with A.Scratch(algo, maxn) as s:
+        for t, a in enumerate(oracle.stress_inputs(seed, trials, maxn)):
+            n = a.shape[0]
+            xs[:n].copy_(torch.from_numpy(a))
+            outs.fill_(int(POISON))
+            incl = (t % 4 == 3)
+            got = solve_on(s, torch, a, incl, x_dev=xs, out_dev=outs)
+            check(got, a, incl, "%s stress seed=%d trial=%d n=%d" % (algo, seed, t, n))
+
+
+LE_KINDS = {
+    "random": lambda N: oracle.glibc_rand(1234, N),
+    "dup16": lambda N: oracle.glibc_rand(99, N) % 16,
+    "dup2": lambda N: oracle.glibc_rand(7, N) % 2,
+    "all-equal": lambda N: np.full(N, 42, dtype=np.int32),
+    "descending": lambda N: (N - np.arange(N)).astype(np.int32),
+    "ascending": lambda N: np.arange(N, dtype=np.int32),
+}
+
+
+def _intmax(N):
+    # srand(5); h[i] = (rand() % 4 == 0) ? INT_MAX : rand()  (two draws/elt
+    # when the first is not a multiple of 4, src/bench_spt_le.cu:122-125)
+    r = oracle.glibc_rand(5, 2 * N + 16)
+    out = np.empty(N, dtype=np.int32)
+    k = 0
+    for i in range(N):
+        if r[k] % 4 == 0:
+            out[i] = 2147483647
+            k += 1
+        else:
+            out[i] = r[k + 1]
+            k += 2
+    return out
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("kind", list(LE_KINDS) + ["intmax"])
+def test_le_matrix(torch_mod, algo, kind):
+    import apsep_amd as A
+    torch = torch_mod
+    N = 1 << 25 if kind != "intmax" else 1 << 21
+    h = LE_KINDS[kind](N) if kind != "intmax" else _intmax(N)
+    h = np.ascontiguousarray(h, dtype=np.int32)
+    x = torch.from_numpy(h).cuda()
+    out = torch.empty(N, dtype=torch.int32, device="cuda")
+    with A.Scratch(algo, N) as s:
+        for n in (N, 1000003, 513, 512, 511, 1):
+            for incl in (True, False):
+                _log(algo, kind, n, "<=" if incl else "<")
+                out.fill_(int(POISON))
+                got = solve_on(s, torch, h[:n], incl, x_dev=x, out_dev=out)
+                check(got, h[:n], incl, "%s %s n=%d incl=%d" % (algo, kind, n, incl))
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+def test_tile_and_level_boundaries(torch_mod, algo):
+    import apsep_amd as A
+    torch = torch_mod
+    T = 2048
+    sizes = [1, 2, 63, 64, 65, 2047, 2048, 2049, 4095, 4096, 4097,
+             64 * T - 1, 64 * T, 64 * T + 1, 64 * 64 * T + 1]
+    nmax = max(sizes)
+    with A.Scratch(algo, nmax) as s:
+        for n in sizes:
+            for kind in ("random", "dup2", "ascending", "descending", "blockdesc"):
+                a = oracle.fill(kind, n, seed=n)
+                _log(algo, "boundary", n, kind)
+                check(solve_on(s, torch, a, False), a, False, "%s n=%d %s" % (algo, n, kind))
+            a = oracle.fill("dup16", n, seed=n + 1)
+            check(solve_on(s, torch, a, True), a, True, "%s n=%d dup16 incl" % (algo, n))
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+def test_adversarial_deep_and_overflow(torch_mod, algo):
+    """Inputs whose open elements need long searches or exceed the recorded
+    open list: a small first element followed by a long descent (every
+    answer is 0), a global descent with periodic dips, block-descending runs,
+    and many equal minima."""
+    import apsep_amd as A
+    torch = torch_mod
+    n = 1 << 22
+    i = np.arange(n, dtype=np.int64)
+    cases = {
+        "zero-then-descent": np.where(i == 0, 0, n - i + 5).astype(np.int32),
+        "descent-with-dips": np.where(i % 9973 == 0, 7, 3 * n - i).astype(np.int32),
+        "blockdesc": oracle.fill("blockdesc", n, seed=5),
+        "sawtooth-desc": ((n - i) % 3000 + (i % 2)).astype(np.int32),
+        "min-plateau": np.where(i % 2048 < 100, 1, 5 + (i % 7)).astype(np.int32),
+        "two-level-descent": (((n - i) // 2048) * 4096 + (i % 2048)).astype(np.int32),
+    }
+    with A.Scratch(algo, n) as s:
+        for name, a in cases.items():
+            for incl in (False, True):
+                _log(algo, name, incl)
+                check(solve_on(s, torch, a, incl), a, incl, "%s %s" % (algo, name))
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("n,kinds", [
+    (1 << 20, ("random", "descending", "ascending")),
+    (1 << 25, ("random", "descending", "ascending")),
+    (1 << 28, ("dup16", "sawtooth", "blockdesc")),
+])
+def test_benchmark_configs(torch_mod, algo, n, kinds):
+    import apsep_amd as A
+    torch = torch_mod
+    x = torch.empty(n, dtype=torch.int32, device="cuda")
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    with A.Scratch(algo, n) as s:
+        for kind in kinds:
+            a = oracle.fill(kind, n, seed=2024)
+            x.copy_(torch.from_numpy(a))
+            out.fill_(int(POISON))
+            _log(algo, "config", n, kind)
+            got = solve_on(s, torch, a, False, x_dev=x, out_dev=out)
+            check(got, a, False, "%s n=%d %s" % (algo, n, kind))
+            # repeat on the same buffers: deterministic, nothing carried over
+            got2 = solve_on(s, torch, a, False, x_dev=x, out_dev=out)
+            assert np.array_equal(got, got2)
