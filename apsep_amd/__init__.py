@@ -35,6 +35,7 @@ EXPORTS = [
     "apsep_status_string", "apsep_version", "apsep_scratch_create",
     "apsep_scratch_destroy", "apsep_scratch_bytes", "apsep_run",
     "apsep_launch", "apsep_run_mode", "apsep_scratch_query",
+    "apsep_scratch_check",
 ]
 
 _lib = None
@@ -75,6 +76,7 @@ def lib() -> ctypes.CDLL:
         L.apsep_launch.argtypes = [ci, vp, vp, i64, ci]
         L.apsep_run_mode.argtypes = [vp, vp, vp, i64, ci, vp]
         L.apsep_scratch_query.argtypes = [vp, ctypes.POINTER(ScratchInfo)]
+        L.apsep_scratch_check.argtypes = [vp, ctypes.POINTER(i32)]
         for f in EXPORTS:
             if f != "apsep_status_string" and f != "apsep_version":
                 getattr(L, f).restype = ci
@@ -138,6 +140,14 @@ class Scratch:
         self.run_ptr(x.data_ptr(), out.data_ptr(), n, stream.cuda_stream, incl)
         return out
 
+    def check(self):
+        """Blocking check of the device-side error word (raises on error)."""
+        e = ctypes.c_int32(0)
+        _check(lib().apsep_scratch_check(self._h, ctypes.byref(e)), "apsep_scratch_check")
+        if e.value:
+            raise ApsepError(ERR_LAUNCH, "device-side error %d (grid barrier timeout)"
+                             % e.value)
+
     def close(self):
         if self._h is not None and self._h.value:
             lib().apsep_scratch_destroy(self._h)
@@ -170,4 +180,5 @@ def solve(x, algo="spt", incl: bool = False, out=None):
     with Scratch(algo, x.numel(), incl) as s:
         s.run(x, out)
         torch.cuda.current_stream(x.device).synchronize()
+        s.check()
     return out

@@ -111,6 +111,14 @@ apsep_status apsep_scratch_query(const apsep_scratch* s, apsep_scratch_info* inf
     return APSEP_OK;
 }
 
+apsep_status apsep_scratch_check(apsep_scratch* s, int32_t* device_error) {
+    if (!s || !device_error) return APSEP_ERR_INVALID_ARG;
+    int code = 0;
+    apsep_status st = single_error(s, &code);
+    *device_error = code;
+    return st;
+}
+
 static apsep_status check_io(const int32_t* in, int32_t* out, int64_t n) {
     if (!in || !out) return APSEP_ERR_INVALID_ARG;
     if (n > INT32_MAX) return APSEP_ERR_TOO_LARGE;
@@ -156,6 +164,11 @@ apsep_status apsep_launch(apsep_algo algo, const int32_t* d_in, int32_t* d_out,
     if (st == APSEP_OK && hipDeviceSynchronize() != hipSuccess) {
         (void)hipGetLastError();
         st = APSEP_ERR_LAUNCH;
+    }
+    if (st == APSEP_OK) {
+        int32_t derr = 0;
+        st = apsep_scratch_check(s, &derr);
+        if (st == APSEP_OK && derr) st = APSEP_ERR_LAUNCH;
     }
     apsep_scratch_destroy(s);
     return st;

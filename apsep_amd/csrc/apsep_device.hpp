@@ -1,33 +1,39 @@
 // apsep_device.hpp -- gfx950 device building blocks shared by every APSEP
-// solver variant (WSTL multi-launch, SPT cooperative, look-back, segmented).
+// solver variant (WSTL multi-launch, SPT cooperative, segmented).
 //
 // Problem (src/apsep.cuh:1-4): out[i] = largest j < i with in[j] < in[i]
 // (strict) or in[j] <= in[i] (INCL), else -1.
 //
 // Work decomposition (wave64 throughout):
-//   tile   = one workgroup of NT = 256 lanes (4 waves) x K = 8 consecutive
-//            elements per lane = 2048 elements, loaded with two dwordx4
+//   tile   = one workgroup of NT = 256 lanes (4 waves) x K = 16 consecutive
+//            elements per lane = 4096 elements, loaded with four dwordx4
 //            loads per lane.
-//   chunk  = 64 consecutive elements (8 lanes); its min is published for the
+//   chunk  = 64 consecutive elements (4 lanes); its min is published for the
 //            inter-tile search.
-//   levels = 64-ary min hierarchy over tile mins (level 0 = tile mins); each
-//            level also stores the exclusive prefix-min of every entry within
-//            its 64-entry group, so the exclusive prefix-min of tile t is the
-//            min over levels of pre[l][t >> 6l].
+//   levels = 64-ary min hierarchy over tile mins.  Level 0 (tile mins) is a
+//            plain array; levels >= 1 hold 64-bit keys (run epoch in the high
+//            word, biased min in the low word).  The tile pass accumulates
+//            level 1 itself with device-scope atomicMin (64 tiles per key),
+//            with no reset between runs and no memory fence: a key from an
+//            older run compares larger and decodes as empty.  Levels >= 2 are
+//            built by a one-workgroup pass only when level 1 exceeds
+//            UPPER_MIN entries; below that, readers fold level 1 directly.
 //
 // Inside a tile (solve_tile):
 //   1. in-lane answers by an unrolled O(K^2) compare/select over registers;
 //   2. Hillis-Steele min scan over the 64 lane-mins of each wave; every
-//      step's window minimum is kept in LDS (win[s][lane] = min over the
-//      2^s lanes ending at lane), which turns the cross-lane search into a
+//      step's window minimum is kept in LDS (win[s][lane] = min over the 2^s
+//      lanes ending at lane), which turns the cross-lane search into a
 //      6-step binary lifting per query;
-//   3. elements still unanswered whose value beats the exclusive prefix-min
-//      of earlier lanes are answered inside the tile (lifting, then the
-//      rightmost qualifying element of the found lane, read from LDS);
+//   3. an element unanswered in its lane is answered inside the tile iff the
+//      exclusive prefix-min of the earlier lanes qualifies: the lane's first
+//      element is checked against the previous lane's 16 values, every other
+//      such element goes through a per-wave LDS queue whose entries all 64
+//      lanes resolve by lifting, one query per lane per round;
 //   4. the rest are the tile's "open" elements: the tile's prefix minima.
-//      They are written as -1 and recorded (first QS positions + count) for
-//      the fix-up phase, which only revisits the ones whose value beats the
-//      exclusive prefix-min of all earlier tiles.
+//      They are written as -1 and recorded (up to QS (position, value) pairs
+//      + count) for the fix-up phase, which only revisits the ones whose
+//      value beats the exclusive prefix-min of all earlier tiles.
 //
 // Invariants relied on (reference reasoning, docs/spt_design.md:140-151 and
 // docs/spt_early_out_regression.md:42-51): open elements of a tile are its
@@ -45,19 +51,27 @@ namespace apsep {
 
 constexpr int WAVE = 64;
 constexpr int NT = 256;            // lanes per tile workgroup
-constexpr int K = 8;               // consecutive elements per lane
-constexpr int TILE = NT * K;       // 2048 elements per tile
+constexpr int K = 16;              // consecutive elements per lane
+constexpr int K4 = K / 4;          // dwordx4 per lane
+constexpr int TILE = NT * K;       // 4096 elements per tile
 constexpr int NWAVE = NT / WAVE;   // 4 waves per tile
 constexpr int CHUNK = 64;          // elements per published chunk min
-constexpr int CPT = TILE / CHUNK;  // 32 chunk mins per tile
-constexpr int QS = 32;             // recorded open positions per tile
+constexpr int LPC = CHUNK / K;     // lanes per chunk (4)
+constexpr int LOG_LPC = 2;
+constexpr int CPT = TILE / CHUNK;  // 64 chunk mins per tile
+constexpr int QS = 32;             // recorded open elements per tile
+constexpr int QCAP = 192;          // per-wave LDS queue of cross-lane queries
 constexpr int FAN = 64;            // hierarchy fan-out
 constexpr int LOG_FAN = 6;
 constexpr int MAX_LEVELS = 5;      // 64^5 tiles >> 2^31 / TILE
+constexpr int UPPER_MIN = 512;     // level-1 size above which levels >= 2 are stored
 constexpr int INF = INT_MAX;
+#define SPT_WAVES_PER_EU 4         // occupancy the persistent kernel is pinned at
 
-static_assert(K == 8, "tile loader assumes two dwordx4 per lane");
-static_assert(CHUNK % K == 0 && (CHUNK / K) == 8, "chunk = 8 lanes");
+static_assert(K == 16 && LPC == 4 && (1 << LOG_LPC) == LPC, "tile shape");
+static_assert(CPT % 32 == 0, "chunk-min search works in blocks of 32");
+
+typedef unsigned long long u64;
 
 // "x qualifies as the answer for query value v"
 template <bool INCL>
@@ -67,30 +81,66 @@ __device__ __forceinline__ bool qual(int x, int v) {
 
 // Hierarchy of tile mins (device pointers + sizes for one solve).
 struct Levels {
+    int nt;                   // tiles
     int nlev;                 // >= 1; the top level has <= 64 entries
+    int upper;                // levels >= 2 stored (level 1 > UPPER_MIN entries)
     int n[MAX_LEVELS];        // entries per level
-    int* mn[MAX_LEVELS];      // mins (mn[0] = tile mins)
-    int* pre[MAX_LEVELS];     // exclusive prefix-min within the 64-group
+    int* mn0;                 // level 0: tile mins
+    u64* key[MAX_LEVELS];     // levels >= 1: epoch-keyed mins (key[0] unused)
+    unsigned epoch;           // this run's tag (never 0 or ~0u)
 };
+
+__device__ __forceinline__ u64 make_key(unsigned epoch, int v) {
+    return ((u64)(~epoch) << 32) | (u64)((unsigned)v ^ 0x80000000u);
+}
+__device__ __forceinline__ int key_value(u64 k, unsigned epoch) {
+    return ((unsigned)(k >> 32) == ~epoch) ? (int)((unsigned)k ^ 0x80000000u) : INF;
+}
 
 // Per-tile summaries written by the tile pass.
 struct TileOut {
-    int* tmin;          // [nt] tile mins (== Levels::mn[0])
     int* cmin;          // [nt * CPT] chunk mins
     int* qcount;        // [nt] number of open elements
-    uint16_t* qlist;    // [nt * QS] tile-relative positions of the first QS
+    int2* qlist;        // [nt * QS] (tile-relative position, value), any order
 };
 
+// LDS of one tile.  Element e = lane*K + i lives in val4[i >> 2][lane],
+// component i & 3: lane-contiguous dwordx4 rows, so every ds_*_b128 of the
+// tile pass is bank-conflict free.
 struct TileSmem {
-    int val[TILE];              // the tile's values
+    int4 val4[K4][NT];          // the tile's values
     int win[6][NT];             // window mins of lane mins, sizes 1..32
+    int incl[NT];               // inclusive prefix-min of lane mins in the wave
     int wmin[NWAVE];            // wave mins
-    int wopen[NWAVE];           // open-element counts per wave
+    int wopen[NWAVE];           // open elements per wave
+    int q[NWAVE][QCAP];         // per-wave queue: query (lane << 4 | slot) -> answer
 };
+
+__device__ __forceinline__ int tile_value(const TileSmem& sm, int e) {
+    const int4 w = sm.val4[(e >> 2) & (K4 - 1)][e >> 4];
+    const int c = e & 3;
+    return c == 0 ? w.x : c == 1 ? w.y : c == 2 ? w.z : w.w;
+}
 
 __device__ __forceinline__ int wave_shfl_up(int x, int d) {
     return __shfl_up(x, d, WAVE);
 }
+
+// ds_bpermute byte addresses of lane - 2^s (own lane below 2^s), computed
+// once per thread: every shift-up of the tile pass is then one LDS-crossbar
+// op plus the combining VALU op.
+struct UpAddr {
+    int a[6];
+    __device__ __forceinline__ UpAddr() {
+        const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+        for (int s = 0; s < 6; s++) a[s] = ((lane >= (1 << s)) ? lane - (1 << s) : lane) << 2;
+    }
+};
+__device__ __forceinline__ int bperm(int addr, int x) {
+    return __builtin_amdgcn_ds_bpermute(addr, x);
+}
+__device__ __forceinline__ int lane63(int x) { return __builtin_amdgcn_readlane(x, WAVE - 1); }
 
 // Inclusive min scan over the wave (Hillis-Steele), lanes below the shift
 // keep their value, so every step is a window clipped at lane 0.
@@ -104,10 +154,23 @@ __device__ __forceinline__ int wave_incl_add(int x) {
     const int lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
     for (int d = 1; d < WAVE; d <<= 1) {
-        int y = wave_shfl_up(x, d);
+        const int y = wave_shfl_up(x, d);
         if (lane >= d) x += y;
     }
     return x;
+}
+
+__device__ __forceinline__ int wave_min(int x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x = min(x, __shfl_xor(x, d, WAVE));
+    return x;
+}
+
+// Intra-wave LDS hand-off: LDS operations of one wave complete in order; this
+// keeps the compiler from moving them across and waits for issued ones.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
 // Rightmost index in [lo, hi] with qual(arr[idx], v), or -1.  Reads aligned
@@ -128,247 +191,548 @@ __device__ __forceinline__ int scan_back(const int* __restrict__ arr, int lo,
     return -1;
 }
 
-// Exclusive prefix-min of tile t over tiles [0, t) (t >= 1).
-__device__ __forceinline__ int tile_prefix(const Levels& L, int t) {
-    int p = INF;
-#pragma unroll
-    for (int l = 0; l < MAX_LEVELS; l++)
-        if (l < L.nlev) p = min(p, L.pre[l][t >> (LOG_FAN * l)]);
-    return p;
+// Same over an epoch-keyed level.
+template <bool INCL>
+__device__ __forceinline__ int scan_back_key(const u64* __restrict__ arr, int lo, int hi,
+                                             int v, unsigned epoch) {
+    for (int i = hi; i >= lo; i--)
+        if (qual<INCL>(key_value(arr[i], epoch), v)) return i;
+    return -1;
 }
 
-// Global index of the nearest j < tile_start(t) with qual(in[j], v).  The
-// caller guarantees one exists (qual(tile_prefix(t), v)).  Ascends the
-// 64-ary hierarchy until an earlier entry qualifies, descends to the tile,
-// then to the chunk, then to the element.
+// Rightmost qualifying index in the aligned 32-entry block [b, b+32) (b a
+// multiple of 4), all loads issued together; -1 if none.
+template <bool INCL>
+__device__ __forceinline__ int last_in_32(const int* __restrict__ arr, int b, int v) {
+    int4 w[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) w[k] = *reinterpret_cast<const int4*>(arr + b + 4 * k);
+    int r = -1;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        r = qual<INCL>(w[k].x, v) ? b + 4 * k + 0 : r;
+        r = qual<INCL>(w[k].y, v) ? b + 4 * k + 1 : r;
+        r = qual<INCL>(w[k].z, v) ? b + 4 * k + 2 : r;
+        r = qual<INCL>(w[k].w, v) ? b + 4 * k + 3 : r;
+    }
+    return r;
+}
+
+// Rightmost qualifying index in the aligned 64-entry block [b, b+64), all
+// 16 loads issued together; -1 if none.
+template <bool INCL>
+__device__ __forceinline__ int last_in_64(const int* __restrict__ arr, int b, int v) {
+    int4 w[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) w[k] = *reinterpret_cast<const int4*>(arr + b + 4 * k);
+    int r = -1;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        r = qual<INCL>(w[k].x, v) ? b + 4 * k + 0 : r;
+        r = qual<INCL>(w[k].y, v) ? b + 4 * k + 1 : r;
+        r = qual<INCL>(w[k].z, v) ? b + 4 * k + 2 : r;
+        r = qual<INCL>(w[k].w, v) ? b + 4 * k + 3 : r;
+    }
+    return r;
+}
+
+// Exclusive prefix-min of tile t over tiles [0, t) (t >= 1); called by a
+// whole wave (uniform t), result uniform.  Per level, the entries before
+// t's ancestor inside the ancestor's 64-group (one coalesced wave load per
+// level, all independent); without stored upper levels, all level-1
+// entries before the ancestor's 64-group (at most UPPER_MIN / 64 loads).
+__device__ __forceinline__ int tile_prefix(const Levels& L, int t) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int g0 = (t >> LOG_FAN) << LOG_FAN;
+    int p = (g0 + lane < t) ? L.mn0[g0 + lane] : INF;
+    if (L.nlev > 1) {
+        const int a1 = t >> LOG_FAN;
+        const int b1 = (a1 >> LOG_FAN) << LOG_FAN;
+        if (b1 + lane < a1) p = min(p, key_value(L.key[1][b1 + lane], L.epoch));
+        if (L.upper) {
+#pragma unroll
+            for (int l = 2; l < MAX_LEVELS; l++) {
+                if (l < L.nlev) {
+                    const int a = t >> (LOG_FAN * l);
+                    const int idx = ((a >> LOG_FAN) << LOG_FAN) + lane;
+                    if (idx < a) p = min(p, key_value(L.key[l][idx], L.epoch));
+                }
+            }
+        } else {
+            for (int i = lane; i < b1; i += WAVE) p = min(p, key_value(L.key[1][i], L.epoch));
+        }
+    }
+    return wave_min(p);
+}
+
+// Nearest tile s < t with qual(tile min, v); the caller guarantees one
+// exists (qual(tile_prefix(t), v)).  Ascends the 64-ary hierarchy until an
+// earlier entry qualifies, then descends to the tile.
+template <bool INCL>
+__device__ int find_tile(const Levels& L, int t, int v) {
+    int found = scan_back<INCL>(L.mn0, (t - 1) & ~(FAN - 1), t - 1, v);
+    if (found >= 0) return found;
+    int x = (t - 1) >> LOG_FAN;  // exclusive bound at level 1
+    if (L.nlev < 2 || x <= 0) return -1;  // unreachable when guaranteed
+    int lvl = 1;
+    if (L.upper) {
+        while (true) {
+            found = scan_back_key<INCL>(L.key[lvl], (x - 1) & ~(FAN - 1), x - 1, v, L.epoch);
+            if (found >= 0) break;
+            x = (x - 1) >> LOG_FAN;
+            lvl++;
+            if (lvl >= L.nlev || x <= 0) return -1;
+        }
+        while (lvl > 1) {
+            lvl--;
+            const int lo = found << LOG_FAN;
+            found = scan_back_key<INCL>(L.key[lvl], lo, min(lo + FAN - 1, L.n[lvl] - 1), v,
+                                        L.epoch);
+        }
+    } else {
+        found = scan_back_key<INCL>(L.key[1], 0, x - 1, v, L.epoch);
+        if (found < 0) return -1;
+    }
+    const int lo = found << LOG_FAN;
+    return scan_back<INCL>(L.mn0, lo, min(lo + FAN - 1, L.n[0] - 1), v);
+}
+
+template <bool INCL>
+__device__ __forceinline__ int find_in_tile(const int* __restrict__ in,
+                                            const int* __restrict__ cmin, int s, int v) {
+    static_assert(CPT == 64 && CHUNK == 64, "one 64-block per level");
+    const int c = last_in_64<INCL>(cmin, s * CPT, v);  // global chunk index
+    return last_in_64<INCL>(in, c * CHUNK, v);
+}
+
 template <bool INCL>
 __device__ int search_before_tile(const int* __restrict__ in, const Levels& L,
                                   const int* __restrict__ cmin, int t, int v) {
-    int lvl = 0, x = t, found = -1;
-    while (true) {
-        const int gstart = ((x - 1) >> LOG_FAN) << LOG_FAN;
-        found = scan_back<INCL>(L.mn[lvl], gstart, x - 1, v);
-        if (found >= 0) break;
-        x = (x - 1) >> LOG_FAN;
-        lvl++;
-        if (lvl >= L.nlev || x <= 0) return -1;  // unreachable when guaranteed
+    const int s = find_tile<INCL>(L, t, v);
+    if (s < 0) return -1;
+    return find_in_tile<INCL>(in, cmin, s, v);
+}
+
+// Rightmost j in [0, K) with qual(lane lt's value j, v), or -1.
+template <bool INCL>
+__device__ __forceinline__ int last_in_lane(const TileSmem& sm, int lt, int v) {
+    int r = -1;
+#pragma unroll
+    for (int q = 0; q < K4; q++) {
+        const int4 w = sm.val4[q][lt];
+        r = qual<INCL>(w.x, v) ? 4 * q + 0 : r;
+        r = qual<INCL>(w.y, v) ? 4 * q + 1 : r;
+        r = qual<INCL>(w.z, v) ? 4 * q + 2 : r;
+        r = qual<INCL>(w.w, v) ? 4 * q + 3 : r;
     }
-    while (lvl > 0) {
-        lvl--;
-        const int lo = found << LOG_FAN;
-        const int hi = min(lo + FAN - 1, L.n[lvl] - 1);
-        found = scan_back<INCL>(L.mn[lvl], lo, hi, v);
+    return r;
+}
+
+// Answer (tile-relative element index) of the query value `val` asked by
+// tile lane qt, given that some lane before qt holds a qualifying element.
+template <bool INCL>
+__device__ __forceinline__ int intile_answer(const TileSmem& sm, int qt, int val) {
+    const int wq = qt >> 6, ql = qt & (WAVE - 1);
+    int wb, p;
+    if (ql > 0 && qual<INCL>(sm.incl[qt - 1], val)) {
+        wb = wq * WAVE;
+        p = ql - 1;
+    } else {
+        int w = wq - 1;  // an earlier wave qualifies; bounded anyway
+        while (w > 0 && !qual<INCL>(sm.wmin[w], val)) w--;
+        wb = w * WAVE;
+        p = WAVE - 1;
     }
-    const int s = found;
-    const int c = scan_back<INCL>(cmin, s * CPT, s * CPT + CPT - 1, v);
-    const int e0 = c * CHUNK;  // == s*TILE + (c - s*CPT)*CHUNK
-    return scan_back<INCL>(in, e0, e0 + CHUNK - 1, v);
+#pragma unroll
+    for (int s = 5; s >= 0; s--)
+        if (!qual<INCL>(sm.win[s][wb + p], val)) p -= 1 << s;
+    const int lt = wb + p;
+    const int j = last_in_lane<INCL>(sm, lt, val);
+    return lt * K + (j < 0 ? 0 : j);
 }
 
 // ---------------------------------------------------------------------------
 // Tile pass: solve tile t inside the workgroup, write every output of the
-// tile (open elements as -1), publish tile min, chunk mins and open list.
-// The caller must __syncthreads() before reusing `sm` for another tile.
+// tile (open elements as -1), publish chunk mins, open list, tile min and
+// the tile's contribution to level 1.  Ends with a workgroup barrier, so
+// `sm` may be reused for the next tile right away.
 // ---------------------------------------------------------------------------
+// This lane's K values of tile t (INF past the end of the array).
+__device__ __forceinline__ void load_lane(const int* __restrict__ in, int n, int t, int* v) {
+    const int base = t * TILE + (int)threadIdx.x * K;
+    if (t * TILE + TILE <= n) {
+#pragma unroll
+        for (int q = 0; q < K4; q++) {
+            const int4 a = *reinterpret_cast<const int4*>(in + base + 4 * q);
+            v[4 * q] = a.x; v[4 * q + 1] = a.y; v[4 * q + 2] = a.z; v[4 * q + 3] = a.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < K; i++) v[i] = (base + i < n) ? in[base + i] : INF;
+    }
+}
+
 template <bool INCL>
-__device__ __forceinline__ void solve_tile(const int* __restrict__ in,
-                                           int* __restrict__ out, int n, int t,
-                                           TileSmem& sm, const TileOut& to) {
+__device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, const int* v,
+                                           TileSmem& sm, const TileOut& to,
+                                           const Levels& L, const UpAddr& up) {
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     const int wave = tid >> 6;
     const int tile_start = t * TILE;
     const int base = tile_start + tid * K;
-    const bool full = tile_start + TILE <= n;
+    // elements of this lane inside the array (the last tile is partial; its
+    // padding is INF and never becomes a query, an open element or an output)
+    const int nvalid = min(max(n - base, 0), K);
+    const unsigned vmask = (nvalid >= K) ? 0xFFFFu : ((1u << nvalid) - 1u);
 
-    int v[K];
-    if (full) {
-        const int4 a = *reinterpret_cast<const int4*>(in + base);
-        const int4 b = *reinterpret_cast<const int4*>(in + base + 4);
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-    } else {
-#pragma unroll
-        for (int i = 0; i < K; i++) v[i] = (base + i < n) ? in[base + i] : INF;
-    }
-
-    // 1. in-lane answers (lane-relative index or -1)
-    int loc[K];
+    // 1. in-lane answers (global index, -1 if none in the lane)
+    int res[K];
 #pragma unroll
     for (int i = 0; i < K; i++) {
-        loc[i] = -1;
+        res[i] = -1;
 #pragma unroll
-        for (int j = 0; j < i; j++)
-            if (qual<INCL>(v[j], v[i])) loc[i] = j;
+        for (int j = 0; j < i; j++) res[i] = qual<INCL>(v[j], v[i]) ? base + j : res[i];
     }
     int lmin = v[0];
 #pragma unroll
     for (int i = 1; i < K; i++) lmin = min(lmin, v[i]);
 
-    *reinterpret_cast<int4*>(&sm.val[tid * K]) = make_int4(v[0], v[1], v[2], v[3]);
-    *reinterpret_cast<int4*>(&sm.val[tid * K + 4]) = make_int4(v[4], v[5], v[6], v[7]);
+#pragma unroll
+    for (int q = 0; q < K4; q++)
+        sm.val4[q][tid] = make_int4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
 
     // 2. window mins of lane mins (Hillis-Steele within the wave)
     int x = lmin, chunk_min = 0;
     sm.win[0][tid] = x;
 #pragma unroll
     for (int s = 1; s <= 6; s++) {
-        x = min(x, wave_shfl_up(x, 1 << (s - 1)));
+        x = min(x, bperm(up.a[s - 1], x));
         if (s < 6) sm.win[s][tid] = x;
-        if (s == 3) chunk_min = x;  // window of 8 lanes = 64 elements
+        if (s == LOG_LPC) chunk_min = x;  // window of LPC lanes = 64 elements
     }
-    const int wexcl = wave_shfl_up(x, 1);  // valid for lane > 0
+    sm.incl[tid] = x;
+    const int wexcl = bperm(up.a[0], x);  // valid for lane > 0
     if (lane == WAVE - 1) sm.wmin[wave] = x;
-    if ((lane & 7) == 7) to.cmin[(size_t)t * CPT + (tid >> 3)] = chunk_min;
+    if ((lane & (LPC - 1)) == LPC - 1) to.cmin[(size_t)t * CPT + (tid >> LOG_LPC)] = chunk_min;
     __syncthreads();
 
     int cross = INF;
 #pragma unroll
     for (int w = 0; w < NWAVE - 1; w++)
         if (w < wave) cross = min(cross, sm.wmin[w]);
+    // exclusive prefix-min of the tile's earlier lanes; meaningful only
+    // when has_prev (INCL needs the distinction: INF is a legal key)
+    const bool has_prev = tid > 0;
+    const int pmx = (lane > 0) ? min(wexcl, cross) : cross;
 
-    // classify lane-unanswered elements
+    // classify lane-unanswered elements (branch-free)
     unsigned qmask = 0, omask = 0;
-    int res[K];
 #pragma unroll
     for (int i = 0; i < K; i++) {
-        res[i] = (loc[i] >= 0) ? base + loc[i] : -1;
-        const bool valid = full || (base + i < n);
-        if (valid && loc[i] < 0) {
-            const bool in_tile = (lane > 0 && qual<INCL>(wexcl, v[i])) ||
-                                 (wave > 0 && qual<INCL>(cross, v[i]));
-            if (in_tile) qmask |= 1u << i;
-            else omask |= 1u << i;
+        const bool rec = res[i] < 0;
+        const bool in_tile = rec && has_prev && qual<INCL>(pmx, v[i]);
+        qmask |= (in_tile ? 1u : 0u) << i;
+        omask |= ((rec && !in_tile) ? 1u : 0u) << i;
+    }
+    qmask &= vmask;
+    omask &= vmask;
+
+    // 3a. the lane's first element (always unanswered in-lane) against the
+    //     previous lane's values
+    if (qmask & 1u) {
+        const int r = last_in_lane<INCL>(sm, tid - 1, v[0]);
+        if (r >= 0) {
+            res[0] = base - K + r;
+            qmask &= ~1u;
         }
     }
 
     // write the tile's outputs (answers found so far; -1 elsewhere)
-    if (full) {
-        *reinterpret_cast<int4*>(out + base) = make_int4(res[0], res[1], res[2], res[3]);
-        *reinterpret_cast<int4*>(out + base + 4) = make_int4(res[4], res[5], res[6], res[7]);
+    if (nvalid == K) {
+#pragma unroll
+        for (int q = 0; q < K4; q++)
+            *reinterpret_cast<int4*>(out + base + 4 * q) =
+                make_int4(res[4 * q], res[4 * q + 1], res[4 * q + 2], res[4 * q + 3]);
     } else {
 #pragma unroll
         for (int i = 0; i < K; i++)
-            if (base + i < n) out[base + i] = res[i];
+            if (i < nvalid) out[base + i] = res[i];
     }
 
-    // 3. cross-lane answers inside the tile (same-thread overwrite of the
-    //    placeholder written above)
-    while (qmask) {
-        const int i = __builtin_ctz(qmask);
-        qmask &= qmask - 1;
-        const int val = sm.val[tid * K + i];
-        int wb, p;
-        if (lane > 0 && qual<INCL>(wexcl, val)) {
-            wb = wave * WAVE;
-            p = lane - 1;
-        } else {
-            // an earlier wave qualifies (qual(cross, val)); bounded anyway
-            int w = wave - 1;
-            while (w > 0 && !qual<INCL>(sm.wmin[w], val)) w--;
-            wb = w * WAVE;
-            p = WAVE - 1;
+    // one scan for both counts: queries (low 16 bits), open (high 16 bits)
+    const int cq = __builtin_popcount(qmask), co = __builtin_popcount(omask);
+    int packed = cq | (co << 16);
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+        const int y = bperm(up.a[s], packed);
+        packed += (lane >= (1 << s)) ? y : 0;
+    }
+    const int ptotal = lane63(packed);
+    const int qoff = (packed & 0xFFFF) - cq, qtotal = ptotal & 0xFFFF;
+    const int ooff = (packed >> 16) - co;
+    if (lane == WAVE - 1) sm.wopen[wave] = ptotal >> 16;
+
+    // 3b. the rest through the wave's queue: compact, resolve one query per
+    //     lane, hand the answers back; the owner overwrites its placeholder
+    //     (same-thread store order).  Entries beyond QCAP are resolved by
+    //     their owner.
+    if (qtotal > 0) {
+        unsigned m = qmask;
+        int k = qoff;
+        while (m) {
+            const int i = __builtin_ctz(m);
+            m &= m - 1;
+            if (k < QCAP) sm.q[wave][k] = (tid << 4) | i;
+            k++;
         }
-#pragma unroll
-        for (int s = 5; s >= 0; s--)
-            if (!qual<INCL>(sm.win[s][wb + p], val)) p -= 1 << s;
-        const int lt = wb + p;
-        const int4 e0 = *reinterpret_cast<const int4*>(&sm.val[lt * K]);
-        const int4 e1 = *reinterpret_cast<const int4*>(&sm.val[lt * K + 4]);
-        const int j = qual<INCL>(e1.w, val) ? 7 : qual<INCL>(e1.z, val) ? 6
-                    : qual<INCL>(e1.y, val) ? 5 : qual<INCL>(e1.x, val) ? 4
-                    : qual<INCL>(e0.w, val) ? 3 : qual<INCL>(e0.z, val) ? 2
-                    : qual<INCL>(e0.y, val) ? 1 : 0;
-        out[base + i] = tile_start + lt * K + j;
+        wave_lds_sync();
+        for (int r = lane; r < min(qtotal, QCAP); r += WAVE) {
+            const int item = sm.q[wave][r];
+            sm.q[wave][r] = tile_start + intile_answer<INCL>(sm, item >> 4, tile_value(sm, item));
+        }
+        wave_lds_sync();
+        m = qmask;
+        k = qoff;
+        while (m) {
+            const int i = __builtin_ctz(m);
+            m &= m - 1;
+            out[base + i] = (k < QCAP) ? sm.q[wave][k]
+                                       : tile_start + intile_answer<INCL>(sm, tid,
+                                                                          tile_value(sm, tid * K + i));
+            k++;
+        }
     }
-
-    // 4. open elements: count, scan, record the first QS positions
-    const int mine = __builtin_popcount(omask);
-    const int incl_cnt = wave_incl_add(mine);
-    if (lane == WAVE - 1) sm.wopen[wave] = incl_cnt;
     __syncthreads();
-    int before = incl_cnt - mine, total = 0;
+
+    // 4. open elements: (position, value) pairs in position order, up to QS
+    int obase = ooff;
 #pragma unroll
-    for (int w = 0; w < NWAVE; w++) {
-        const int c = sm.wopen[w];
-        if (w < wave) before += c;
-        total += c;
-    }
-    unsigned om = omask;
-    while (om && before < QS) {
-        const int i = __builtin_ctz(om);
-        om &= om - 1;
-        to.qlist[(size_t)t * QS + before] = (uint16_t)(tid * K + i);
-        before++;
+    for (int w = 0; w < NWAVE - 1; w++)
+        if (w < wave) obase += sm.wopen[w];
+    if (omask && obase < QS) {
+#pragma unroll
+        for (int i = 0; i < K; i++)
+            if (omask & (1u << i)) {
+                if (obase < QS)
+                    to.qlist[(size_t)t * QS + obase] = make_int2(tid * K + i, v[i]);
+                obase++;
+            }
     }
     if (tid == 0) {
-        int m = sm.wmin[0];
+        int m = sm.wmin[0], total = sm.wopen[0];
 #pragma unroll
-        for (int w = 1; w < NWAVE; w++) m = min(m, sm.wmin[w]);
-        to.tmin[t] = m;
+        for (int w = 1; w < NWAVE; w++) {
+            m = min(m, sm.wmin[w]);
+            total += sm.wopen[w];
+        }
+        L.mn0[t] = m;
         to.qcount[t] = total;
+        if (L.nlev > 1) atomicMin(&L.key[1][t >> LOG_FAN], make_key(L.epoch, m));
     }
+    __syncthreads();  // sm is reused by the next tile
+}
+
+// Solve tile t (loads included) in the calling workgroup.
+template <bool INCL>
+__device__ __forceinline__ void tile_step(const int* __restrict__ in, int* __restrict__ out,
+                                          int n, int t, TileSmem& sm, const TileOut& to,
+                                          const Levels& L, const UpAddr& up) {
+    int v[K];
+    load_lane(in, n, t, v);
+    solve_tile<INCL>(out, n, t, v, sm, to, L, up);
 }
 
 // ---------------------------------------------------------------------------
-// Hierarchy: one wave builds one 64-entry group of level l (exclusive
-// prefix within the group, group min into level l+1).
+// Levels >= 2 (only when level 1 exceeds UPPER_MIN entries): one workgroup
+// builds them from the level below after the tile pass.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void build_group(const int* __restrict__ mn, int* __restrict__ pre,
-                                            int* __restrict__ up, int n, int g) {
+__device__ __forceinline__ void build_upper_levels(const Levels& L) {
     const int lane = threadIdx.x & (WAVE - 1);
-    const int idx = g * FAN + lane;
-    const int x = (idx < n) ? mn[idx] : INF;
-    const int inc = wave_incl_min(x);
-    int ex = wave_shfl_up(inc, 1);
-    if (lane == 0) ex = INF;
-    if (idx < n) pre[idx] = ex;
-    if (up && lane == WAVE - 1) up[g] = inc;
+    const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    for (int l = 2; l < L.nlev; l++) {
+        for (int g = wv; g < L.n[l]; g += nwv) {
+            const int idx = g * FAN + lane;
+            const int x = idx < L.n[l - 1] ? key_value(L.key[l - 1][idx], L.epoch) : INF;
+            const int m = wave_min(x);
+            if (lane == 0) L.key[l][g] = make_key(L.epoch, m);
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Fix-up of tile t by one wave: answer the open elements whose value beats
-// the exclusive prefix-min of all earlier tiles.
+// the exclusive prefix-min of all earlier tiles.  One batch of independent
+// loads brings the prefix inputs, the open list, the 64 tile mins before t
+// and tile t-1's 64 chunk mins (the latter two into the wave's LDS window);
+// the common case (answer in tile t-1) then needs one round of element
+// loads.  `win` is this wave's 128-int LDS area.
 // ---------------------------------------------------------------------------
 template <bool INCL>
 __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __restrict__ out,
                                          int n, int t, const Levels& L,
                                          const int* __restrict__ cmin,
                                          const int* __restrict__ qcount,
-                                         const uint16_t* __restrict__ qlist) {
+                                         const int2* __restrict__ qlist, int* win) {
+    static_assert(CPT == WAVE, "one chunk min of tile t-1 per lane");
     if (t == 0) return;
     const int lane = threadIdx.x & (WAVE - 1);
     const int tile_start = t * TILE;
-    const int P = tile_prefix(L, t);
-    const int v1 = in[tile_start];
-    if (!qual<INCL>(P, v1)) return;  // every open element answers -1
+    const int tw = t - WAVE + lane;  // tile whose min this lane holds
+    const int mw = tw >= 0 ? L.mn0[tw] : INF;
+    const int cw = cmin[(size_t)(t - 1) * CPT + lane];
     const int cnt = qcount[t];
-    const int nq = min(cnt, QS);
-    if (lane < nq) {
-        const int pos = tile_start + qlist[(size_t)t * QS + lane];
-        const int v = in[pos];
-        if (qual<INCL>(P, v)) out[pos] = search_before_tile<INCL>(in, L, cmin, t, v);
-    }
-    if (cnt > QS) {
-        // open elements beyond the recorded ones: rescan the tile's tail.
-        // The prefix-min of the tile up to the last recorded open element is
-        // that element's value (open elements are prefix minima).
-        const int lastpos = tile_start + qlist[(size_t)t * QS + QS - 1];
-        int rm = in[lastpos];
-        const int tile_end = min(tile_start + TILE, n);
-        for (int p0 = lastpos + 1; p0 < tile_end && qual<INCL>(P, rm); p0 += WAVE) {
-            const int idx = p0 + lane;
-            const bool valid = idx < tile_end;
-            const int xv = valid ? in[idx] : INF;
-            const int inc = wave_incl_min(xv);
-            int ex = wave_shfl_up(inc, 1);
-            if (lane == 0) ex = INF;
-            ex = min(ex, rm);
-            if (valid && !qual<INCL>(ex, xv) && qual<INCL>(P, xv))
-                out[idx] = search_before_tile<INCL>(in, L, cmin, t, xv);
-            rm = min(rm, __shfl(inc, WAVE - 1, WAVE));
+    const int v1 = in[tile_start];
+    int2 e = make_int2(0, INF);
+    if (lane < min(cnt, QS)) e = qlist[(size_t)t * QS + lane];
+    // exclusive prefix-min of tile t: level 0 from the window, upper levels
+    // from the keys
+    int p = (tw >= ((t >> LOG_FAN) << LOG_FAN)) ? mw : INF;
+    if (L.nlev > 1) {
+        const int a1 = t >> LOG_FAN;
+        const int b1 = (a1 >> LOG_FAN) << LOG_FAN;
+        if (b1 + lane < a1) p = min(p, key_value(L.key[1][b1 + lane], L.epoch));
+        if (L.upper) {
+#pragma unroll
+            for (int l = 2; l < MAX_LEVELS; l++) {
+                if (l < L.nlev) {
+                    const int a = t >> (LOG_FAN * l);
+                    const int idx = ((a >> LOG_FAN) << LOG_FAN) + lane;
+                    if (idx < a) p = min(p, key_value(L.key[l][idx], L.epoch));
+                }
+            }
+        } else {
+            for (int i = lane; i < b1; i += WAVE) p = min(p, key_value(L.key[1][i], L.epoch));
         }
     }
+    const int P = wave_min(p);
+    if (!qual<INCL>(P, v1)) return;  // every open element answers -1
+    win[lane] = mw;
+    win[WAVE + lane] = cw;
+    wave_lds_sync();
+    if (cnt <= QS) {
+        if (lane < cnt && qual<INCL>(P, e.y)) {
+            const int v = e.y;
+            // tile: the window first (tiles t-64 .. t-1), then the hierarchy
+            int s = -1;
+            for (int i = WAVE - 4; i >= 0 && s < 0; i -= 4) {
+                const int4 w = *reinterpret_cast<const int4*>(&win[i]);
+                s = qual<INCL>(w.w, v) ? i + 3 : qual<INCL>(w.z, v) ? i + 2
+                  : qual<INCL>(w.y, v) ? i + 1 : qual<INCL>(w.x, v) ? i : -1;
+            }
+            // window entries of tiles < 0 hold INF, which never qualifies
+            // except under INCL with v == INF: exclude them explicitly
+            if (s >= 0 && t - WAVE + s < 0) s = -1;
+            int c;
+            if (s == WAVE - 1) {
+                c = -1;
+                for (int i = WAVE - 4; i >= 0 && c < 0; i -= 4) {
+                    const int4 w = *reinterpret_cast<const int4*>(&win[WAVE + i]);
+                    c = qual<INCL>(w.w, v) ? i + 3 : qual<INCL>(w.z, v) ? i + 2
+                      : qual<INCL>(w.y, v) ? i + 1 : qual<INCL>(w.x, v) ? i : -1;
+                }
+                c += (t - 1) * CPT;
+            } else {
+                const int st = (s >= 0) ? t - WAVE + s : find_tile<INCL>(L, t - WAVE, v);
+                c = last_in_64<INCL>(cmin, st * CPT, v);
+            }
+            out[tile_start + e.x] = last_in_64<INCL>(in, c * CHUNK, v);
+        }
+        return;
+    }
+    // more open elements than recorded: rescan the tile.  Open elements are
+    // the tile's prefix minima: an element is open iff no earlier element of
+    // the tile qualifies for it (running prefix-min rm).
+    int rm = INF;
+    bool have = false;  // rm holds a real element (INCL: INF is a legal key)
+    const int tile_end = min(tile_start + TILE, n);
+    for (int p0 = tile_start; p0 < tile_end; p0 += WAVE) {
+        const int idx = p0 + lane;
+        const bool valid = idx < tile_end;
+        const int xv = valid ? in[idx] : INF;
+        const int inc = wave_incl_min(xv);
+        int ex = wave_shfl_up(inc, 1);
+        const bool ex_real = lane > 0 || have;
+        ex = (lane == 0) ? rm : min(ex, rm);
+        const bool open = valid && !(ex_real && qual<INCL>(ex, xv));
+        if (open && qual<INCL>(P, xv))
+            out[idx] = search_before_tile<INCL>(in, L, cmin, t, xv);
+        rm = min(rm, __shfl(inc, WAVE - 1, WAVE));
+        have = true;
+        if (!qual<INCL>(P, rm)) break;  // later open elements are <= rm
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Grid barrier for persistent cooperative launches.  Arrivals are spread
+// over 64 sub-counters (one 128-byte line each); the last arriver of a
+// sub-group arrives at the root; the last root arriver bumps the generation
+// word (its own line).  The spin is bounded: a grid that is not fully
+// co-resident (or any other stall) raises `err` and lets every workgroup
+// leave instead of hanging the device.
+// ---------------------------------------------------------------------------
+constexpr int BAR_SUB = 64;
+constexpr int BAR_LINE = 32;  // uints per 128-byte line
+
+struct GridBarrier {
+    unsigned gen;
+    int err;
+    unsigned pad0[BAR_LINE - 2];
+    unsigned root;
+    unsigned pad1[BAR_LINE - 1];
+    unsigned sub[BAR_SUB * BAR_LINE];
+};
+
+constexpr unsigned BARRIER_SPIN_LIMIT = 1u << 22;  // >= 1 s at >= 250 ns per poll
+
+// Generation a launch starts from.  The counter persists across launches;
+// it cannot advance before every workgroup has arrived at the first
+// barrier, and each workgroup reads it before arriving, so all workgroups
+// of a launch read the same value.
+__device__ __forceinline__ unsigned grid_barrier_start(GridBarrier* b) {
+    return __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns false (uniformly per workgroup) if the barrier timed out.
+__device__ __forceinline__ bool grid_barrier(GridBarrier* b, unsigned nblocks,
+                                             unsigned& gen) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __threadfence();  // publish this block's writes (agent scope)
+        const unsigned g = blockIdx.x % BAR_SUB;
+        const unsigned nsub = nblocks < (unsigned)BAR_SUB ? nblocks : (unsigned)BAR_SUB;
+        const unsigned gsz = nblocks / BAR_SUB + (g < nblocks % BAR_SUB ? 1u : 0u);
+        bool release = false;
+        if (atomicAdd(&b->sub[g * BAR_LINE], 1u) == gsz - 1) {
+            __hip_atomic_store(&b->sub[g * BAR_LINE], 0u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            if (atomicAdd(&b->root, 1u) == nsub - 1) {
+                __hip_atomic_store(&b->root, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                release = true;
+            }
+        }
+        if (release) {
+            __hip_atomic_store(&b->gen, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned spins = 0;
+            while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (++spins > BARRIER_SPIN_LIMIT) {
+                    __hip_atomic_store(&b->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(4);
+            }
+        }
+        __threadfence();  // acquire the other blocks' writes
+        s_ok = ok;
+    }
+    __syncthreads();
+    gen++;
+    return s_ok != 0;
 }
 
 }  // namespace apsep

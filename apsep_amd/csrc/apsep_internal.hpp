@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "../../include/apsep/apsep.h"
+#include "apsep_device.hpp"
 
 struct apsep_scratch {
     apsep_algo algo;      // resolved (never AUTO)
@@ -16,12 +17,14 @@ struct apsep_scratch {
     int nt_cap;
     void* base;
     int64_t bytes;
-    int* lv_mn[5];
-    int* lv_pre[5];
+    int* mn0;                   // tile mins
+    unsigned long long* key[5]; // epoch-keyed levels 1..
+    unsigned epoch;             // next run's epoch
     int* cmin;
     int* qcount;
-    uint16_t* qlist;
+    int2* qlist;
     int coop_grid;        // SPT persistent grid size (occupancy x CUs)
+    apsep::GridBarrier* bar;
 };
 
 namespace apsep {
@@ -31,5 +34,6 @@ int64_t single_scratch_bytes(int64_t n_cap);
 apsep_status single_scratch_init(apsep_scratch* s);
 apsep_status single_run(apsep_scratch* s, const int32_t* in, int32_t* out, int n,
                         int incl, hipStream_t st);
+apsep_status single_error(apsep_scratch* s, int* code);
 
 }  // namespace apsep

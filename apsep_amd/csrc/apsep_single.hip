@@ -1,16 +1,13 @@
 // apsep_single.hip -- single-array APSEP solvers for gfx950 and the C ABI
 // that drives them (include/apsep/apsep.h).
 //
-//   WSTL     tile pass -> one 64-ary level build per hierarchy level ->
-//            fix-up pass (multi-launch; replaces runWSTL,
-//            src/apsep.cuh:2548-2596)
-//   SPT      the same three phases in one cooperative persistent launch,
-//            separated by grid barriers (replaces runSPT,
-//            src/apsep.cuh:3052-3067)
+//   WSTL     tile pass -> level pass (64-ary tile-min hierarchy) -> fix-up
+//            pass (multi-launch; replaces runWSTL, src/apsep.cuh:2548-2596)
+//   SPT      the same phases in one cooperative persistent launch with two
+//            grid barriers (replaces runSPT, src/apsep.cuh:3052-3067)
 //
 // All device work is integer compare/min and index arithmetic; no MFMA.
 #include <hip/hip_runtime.h>
-#include <hip/hip_cooperative_groups.h>
 #include <stdint.h>
 #include <string.h>
 #include <stdlib.h>
@@ -20,67 +17,66 @@
 #include "apsep_internal.hpp"
 #include "../../include/apsep/apsep.h"
 
-namespace cg = cooperative_groups;
-
 namespace apsep {
 
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
 
+// WSTL pass 1: one workgroup per tile (also accumulates level 1).
 template <bool INCL>
 __global__ __launch_bounds__(NT) void k_tile_pass(const int* __restrict__ in,
                                                   int* __restrict__ out, int n,
-                                                  TileOut to) {
+                                                  TileOut to, Levels L) {
     __shared__ TileSmem sm;
-    solve_tile<INCL>(in, out, n, (int)blockIdx.x, sm, to);
+    const UpAddr up;
+    tile_step<INCL>(in, out, n, (int)blockIdx.x, sm, to, L, up);
 }
 
-__global__ __launch_bounds__(256) void k_build_level(const int* __restrict__ mn,
-                                                     int* __restrict__ pre,
-                                                     int* __restrict__ up, int n) {
-    const int g = (int)(blockIdx.x * (blockDim.x / WAVE) + (threadIdx.x >> 6));
-    if (g * FAN >= n) return;
-    build_group(mn, pre, up, n, g);
-}
+// WSTL pass 2 (only when level 1 exceeds UPPER_MIN entries): levels >= 2.
+__global__ __launch_bounds__(256) void k_upper_levels(Levels L) { build_upper_levels(L); }
 
+// WSTL pass 3: one wave per tile.
 template <bool INCL>
 __global__ __launch_bounds__(256) void k_fix_pass(const int* __restrict__ in,
                                                   int* __restrict__ out, int n,
-                                                  int nt, Levels L,
+                                                  Levels L,
                                                   const int* __restrict__ cmin,
                                                   const int* __restrict__ qcount,
-                                                  const uint16_t* __restrict__ qlist) {
+                                                  const int2* __restrict__ qlist) {
+    __shared__ int win[256 / WAVE][2 * WAVE];
     const int t = (int)(blockIdx.x * (blockDim.x / WAVE) + (threadIdx.x >> 6));
-    if (t >= nt) return;
-    fix_tile<INCL>(in, out, n, t, L, cmin, qcount, qlist);
+    if (t >= L.nt) return;
+    fix_tile<INCL>(in, out, n, t, L, cmin, qcount, qlist, win[threadIdx.x >> 6]);
 }
 
-// SPT: one cooperative persistent launch.  Phase 1 grid-strides over tiles,
-// each level of the hierarchy is one grid-strided wave pass, phase 3
-// grid-strides waves over tiles.  Grid barriers: 1 + nlev.
+// SPT: one persistent launch of fully co-resident workgroups (launched
+// cooperatively): tile phase, grid barrier, [upper levels by workgroup 0,
+// grid barrier,] fix-up phase, all grid-strided.  The occupancy attribute
+// guarantees SPT_WAVES_PER_EU waves per SIMD (that many four-wave
+// workgroups per CU) and the host never launches more per CU, so the grid is
+// co-resident whatever register budget the compiler picks.
 template <bool INCL>
-__global__ __launch_bounds__(NT) void k_spt(const int* __restrict__ in,
-                                            int* __restrict__ out, int n, int nt,
-                                            TileOut to, Levels L) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SPT_WAVES_PER_EU)))
+void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
+           TileOut to, Levels L, GridBarrier* bar) {
     __shared__ TileSmem sm;
-    cg::grid_group grid = cg::this_grid();
-    for (int t = (int)blockIdx.x; t < nt; t += (int)gridDim.x) {
-        solve_tile<INCL>(in, out, n, t, sm, to);
-        __syncthreads();
-    }
+    __shared__ int win[NWAVE][2 * WAVE];
+    unsigned gen = grid_barrier_start(bar);
+    const int nt = L.nt;
+    const UpAddr up;
+    for (int t = (int)blockIdx.x; t < nt; t += (int)gridDim.x)
+        tile_step<INCL>(in, out, n, t, sm, to, L, up);
     if (nt <= 1) return;  // uniform across the grid
-    grid.sync();
+    if (!grid_barrier(bar, gridDim.x, gen)) return;
+    if (L.upper) {
+        if (blockIdx.x == 0) build_upper_levels(L);
+        if (!grid_barrier(bar, gridDim.x, gen)) return;
+    }
     const int gw = (int)(blockIdx.x * NWAVE + (threadIdx.x >> 6));
     const int nw = (int)(gridDim.x * NWAVE);
-    for (int l = 0; l < L.nlev; l++) {
-        const int groups = (L.n[l] + FAN - 1) / FAN;
-        int* up = (l + 1 < L.nlev) ? L.mn[l + 1] : nullptr;
-        for (int g = gw; g < groups; g += nw) build_group(L.mn[l], L.pre[l], up, L.n[l], g);
-        grid.sync();
-    }
     for (int t = gw; t < nt; t += nw)
-        fix_tile<INCL>(in, out, n, t, L, to.cmin, to.qcount, to.qlist);
+        fix_tile<INCL>(in, out, n, t, L, to.cmin, to.qcount, to.qlist, win[threadIdx.x >> 6]);
 }
 
 // ---------------------------------------------------------------------------
@@ -104,12 +100,13 @@ int64_t single_scratch_bytes(int64_t n_cap) {
     const int nt = (int)((n_cap + TILE - 1) / TILE);
     int sz[MAX_LEVELS];
     const int nl = level_sizes(nt < 1 ? 1 : nt, sz);
-    int64_t b = 0;
-    for (int l = 0; l < nl; l++) b += 2 * (round4(sz[l]) + 4) * 4;
+    int64_t b = (round4(sz[0]) + 4) * 4;             // tile mins
+    for (int l = 1; l < nl; l++) b += (round4(sz[l]) + 4) * 8;  // keyed levels
     b += (round4((int64_t)nt * CPT) + 4) * 4;       // cmin
     b += (round4(nt) + 4) * 4;                        // qcount
-    b += ((int64_t)nt * QS + 8) * 2;                  // qlist
-    return b + 256 * 8;                               // alignment slack
+    b += ((int64_t)nt * QS + 2) * 8;                  // qlist
+    b += (int64_t)sizeof(GridBarrier);                // grid barrier
+    return b + 256 * 10;                              // alignment slack
 }
 
 static void* carve(char*& p, int64_t bytes) {
@@ -131,17 +128,24 @@ apsep_status single_scratch_init(apsep_scratch* s) {
         return APSEP_ERR_ALLOC;
     }
     char* p = (char*)s->base;
-    for (int l = 0; l < nl; l++) {
-        s->lv_mn[l] = (int*)carve(p, (round4(sz[l]) + 4) * 4);
-        s->lv_pre[l] = (int*)carve(p, (round4(sz[l]) + 4) * 4);
-    }
+    s->mn0 = (int*)carve(p, (round4(sz[0]) + 4) * 4);
+    for (int l = 1; l < nl; l++) s->key[l] = (u64*)carve(p, (round4(sz[l]) + 4) * 8);
+    s->epoch = 1;
     s->cmin = (int*)carve(p, (round4((int64_t)nt * CPT) + 4) * 4);
     s->qcount = (int*)carve(p, (round4(nt) + 4) * 4);
-    s->qlist = (uint16_t*)carve(p, ((int64_t)nt * QS + 8) * 2);
+    s->qlist = (int2*)carve(p, ((int64_t)nt * QS + 2) * 8);
+    s->bar = (GridBarrier*)carve(p, (int64_t)sizeof(GridBarrier));
+    if ((char*)p > (char*)s->base + s->bytes) return APSEP_ERR_ALLOC;  // sizing bug guard
     if (hipMemset(s->base, 0, (size_t)s->bytes) != hipSuccess) {
         (void)hipGetLastError();
         return APSEP_ERR_LAUNCH;
     }
+    // keys start above every real key (a fresh scratch holds no run)
+    for (int l = 1; l < nl; l++)
+        if (hipMemset(s->key[l], 0xFF, (size_t)(round4(sz[l]) + 4) * 8) != hipSuccess) {
+            (void)hipGetLastError();
+            return APSEP_ERR_LAUNCH;
+        }
 
     if (s->algo == APSEP_ALGO_SPT) {
         // the scratch serves both predicates: size for the tighter one
@@ -150,6 +154,9 @@ apsep_status single_scratch_init(apsep_scratch* s) {
         if (e == hipSuccess)
             e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_incl, k_spt<true>, NT, 0);
         if (per_cu_incl < per_cu) per_cu = per_cu_incl;
+        // never above what the waves-per-EU attribute guarantees
+        const int pinned = SPT_WAVES_PER_EU * 4 / NWAVE;
+        if (per_cu > pinned) per_cu = pinned;
         if (e != hipSuccess || per_cu <= 0) {
             (void)hipGetLastError();
             return APSEP_ERR_UNSUPPORTED;
@@ -164,14 +171,28 @@ apsep_status single_scratch_init(apsep_scratch* s) {
     return APSEP_OK;
 }
 
-static void fill_levels(const apsep_scratch* s, int nt, Levels& L) {
+// Levels for a run over nt tiles; advances the scratch's run epoch.
+static apsep_status fill_levels(apsep_scratch* s, int nt, Levels& L, hipStream_t st) {
     int sz[MAX_LEVELS];
+    L.nt = nt;
     L.nlev = level_sizes(nt, sz);
+    L.mn0 = s->mn0;
+    L.upper = (L.nlev > 2 && sz[1] > UPPER_MIN) ? 1 : 0;
     for (int l = 0; l < MAX_LEVELS; l++) {
         L.n[l] = l < L.nlev ? sz[l] : 0;
-        L.mn[l] = l < L.nlev ? s->lv_mn[l] : nullptr;
-        L.pre[l] = l < L.nlev ? s->lv_pre[l] : nullptr;
+        L.key[l] = (l >= 1 && l < L.nlev) ? s->key[l] : nullptr;
     }
+    if (s->epoch >= 0xFFFFFFF0u) {
+        // epoch wrap (after ~4e9 runs): clear every key so none can alias
+        int szc[MAX_LEVELS];
+        const int nlc = level_sizes(s->nt_cap, szc);
+        for (int l = 1; l < nlc; l++)
+            if (hipMemsetAsync(s->key[l], 0xFF, (size_t)szc[l] * 8, st) != hipSuccess)
+                return APSEP_ERR_LAUNCH;
+        s->epoch = 1;
+    }
+    L.epoch = s->epoch++;
+    return APSEP_OK;
 }
 
 template <bool INCL>
@@ -179,19 +200,15 @@ static apsep_status run_wstl(apsep_scratch* s, const int* in, int* out, int n,
                              hipStream_t st) {
     const int nt = (n + TILE - 1) / TILE;
     Levels L;
-    fill_levels(s, nt, L);
-    TileOut to{s->lv_mn[0], s->cmin, s->qcount, s->qlist};
-    hipLaunchKernelGGL(k_tile_pass<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to);
+    if (fill_levels(s, nt, L, st) != APSEP_OK) return APSEP_ERR_LAUNCH;
+    TileOut to{s->cmin, s->qcount, s->qlist};
+    hipLaunchKernelGGL(k_tile_pass<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L);
     if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
     if (nt <= 1) return APSEP_OK;
-    for (int l = 0; l < L.nlev; l++) {
-        const int groups = (L.n[l] + FAN - 1) / FAN;
-        int* up = (l + 1 < L.nlev) ? L.mn[l + 1] : nullptr;
-        hipLaunchKernelGGL(k_build_level, dim3((groups + 3) / 4), dim3(256), 0, st,
-                           L.mn[l], L.pre[l], up, L.n[l]);
-    }
+    if (L.upper)
+        hipLaunchKernelGGL(k_upper_levels, dim3(1), dim3(256), 0, st, L);
     hipLaunchKernelGGL(k_fix_pass<INCL>, dim3((nt + 3) / 4), dim3(256), 0, st, in, out,
-                       n, nt, L, s->cmin, s->qcount, s->qlist);
+                       n, L, s->cmin, s->qcount, s->qlist);
     if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
     return APSEP_OK;
 }
@@ -199,13 +216,14 @@ static apsep_status run_wstl(apsep_scratch* s, const int* in, int* out, int n,
 template <bool INCL>
 static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
                             hipStream_t st) {
-    int nt = (n + TILE - 1) / TILE;
+    const int nt = (n + TILE - 1) / TILE;
     Levels L;
-    fill_levels(s, nt, L);
-    TileOut to{s->lv_mn[0], s->cmin, s->qcount, s->qlist};
+    if (fill_levels(s, nt, L, st) != APSEP_OK) return APSEP_ERR_LAUNCH;
+    TileOut to{s->cmin, s->qcount, s->qlist};
     int grid = s->coop_grid < nt ? s->coop_grid : nt;
     if (grid < 1) grid = 1;
-    void* args[] = {(void*)&in, (void*)&out, (void*)&n, (void*)&nt, (void*)&to, (void*)&L};
+    GridBarrier* bar = s->bar;
+    void* args[] = {(void*)&in, (void*)&out, (void*)&n, (void*)&to, (void*)&L, (void*)&bar};
     hipError_t e = hipLaunchCooperativeKernel((const void*)k_spt<INCL>, dim3(grid), dim3(NT),
                                               args, 0, st);
     if (e != hipSuccess) {
@@ -225,6 +243,26 @@ apsep_status single_run(apsep_scratch* s, const int32_t* in, int32_t* out, int n
     default:
         return APSEP_ERR_INVALID_ARG;
     }
+}
+
+apsep_status single_error(apsep_scratch* s, int* code) {
+    *code = 0;
+    if (!s->bar) return APSEP_OK;
+    int err = 0;
+    if (hipMemcpy(&err, &s->bar->err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+        (void)hipGetLastError();
+        return APSEP_ERR_LAUNCH;
+    }
+    if (err) {
+        // a timed-out barrier leaves its counts behind: reset the whole state
+        if (hipMemset(s->bar, 0, sizeof(GridBarrier)) != hipSuccess ||
+            hipDeviceSynchronize() != hipSuccess) {
+            (void)hipGetLastError();
+            return APSEP_ERR_LAUNCH;
+        }
+    }
+    *code = err;
+    return APSEP_OK;
 }
 
 }  // namespace apsep

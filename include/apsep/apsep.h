@@ -110,6 +110,13 @@ typedef struct apsep_scratch_info {
 
 apsep_status apsep_scratch_query(const apsep_scratch* s, apsep_scratch_info* info);
 
+/* Blocking check of the scratch's device-side error word, then clears it.
+ * *device_error != 0 means a run on this scratch was cut short on the
+ * device (a persistent launch whose workgroups were not all co-resident
+ * times out at its grid barrier instead of hanging); its output is invalid.
+ * No counterpart in the reference, which has no device-side error path. */
+apsep_status apsep_scratch_check(apsep_scratch* s, int32_t* device_error);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,7 @@ def solve_on(scratch, torch, host, incl=None, x_dev=None, out_dev=None):
         if out_dev is None else out_dev
     scratch.run(x, out, n=n, incl=incl)
     torch.cuda.synchronize()
+    scratch.check()
     return out[:n].cpu().numpy()
 
 
@@ -140,8 +141,10 @@ def test_le_matrix(torch_mod, algo, kind):
 def test_tile_and_level_boundaries(torch_mod, algo):
     import apsep_amd as A
     torch = torch_mod
-    T = 2048
-    sizes = [1, 2, 63, 64, 65, 2047, 2048, 2049, 4095, 4096, 4097,
+    with A.Scratch(algo, 16) as s0:
+        T = s0.info.tile_elems
+    # tile, 64-tile group and 4096-tile group edges
+    sizes = [1, 2, 63, 64, 65, T - 1, T, T + 1, 2 * T - 1, 2 * T, 2 * T + 1,
              64 * T - 1, 64 * T, 64 * T + 1, 64 * 64 * T + 1]
     nmax = max(sizes)
     with A.Scratch(algo, nmax) as s:
