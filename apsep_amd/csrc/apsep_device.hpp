@@ -60,6 +60,7 @@ constexpr int LPC = CHUNK / K;     // lanes per chunk (4)
 constexpr int LOG_LPC = 2;
 constexpr int CPT = TILE / CHUNK;  // 64 chunk mins per tile
 constexpr int QS = 32;             // recorded open elements per tile
+constexpr int RS = 32;             // recorded right-spine entries per tile
 constexpr int QCAP = 192;          // per-wave LDS queue of cross-lane queries
 constexpr int FAN = 64;            // hierarchy fan-out
 constexpr int LOG_FAN = 6;
@@ -98,10 +99,18 @@ __device__ __forceinline__ int key_value(u64 k, unsigned epoch) {
 }
 
 // Per-tile summaries written by the tile pass.
+//   The right spine of a tile is its strict suffix minima (elements smaller
+//   than everything after them in the tile).  The nearest qualifying element
+//   of a later tile's query inside tile s is always on s's right spine: a
+//   later element at most as large would be nearer.  Spine values decrease
+//   with rank (rank 0 = the tile's last element), so the answer is the
+//   lowest-ranked qualifying entry.
 struct TileOut {
     int* cmin;          // [nt * CPT] chunk mins
     int* qcount;        // [nt] number of open elements
-    int2* qlist;        // [nt * QS] (tile-relative position, value), any order
+    int2* qlist;        // [nt * QS] (tile-relative position, value)
+    int* scount;        // [nt] number of right-spine entries
+    int2* spine;        // [nt * RS] (tile-relative position, value) by rank
 };
 
 // LDS of one tile.  Element e = lane*K + i lives in val4[i >> 2][lane],
@@ -113,6 +122,7 @@ struct TileSmem {
     int incl[NT];               // inclusive prefix-min of lane mins in the wave
     int wmin[NWAVE];            // wave mins
     int wopen[NWAVE];           // open elements per wave
+    int wspine[NWAVE];          // right-spine entries per wave
     int q[NWAVE][QCAP];         // per-wave queue: query (lane << 4 | slot) -> answer
 };
 
@@ -130,11 +140,15 @@ __device__ __forceinline__ int wave_shfl_up(int x, int d) {
 // once per thread: every shift-up of the tile pass is then one LDS-crossbar
 // op plus the combining VALU op.
 struct UpAddr {
-    int a[6];
+    int a[6];   // lane - 2^s (own lane below)
+    int d[6];   // lane + 2^s (own lane above 63 - 2^s)
     __device__ __forceinline__ UpAddr() {
         const int lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
-        for (int s = 0; s < 6; s++) a[s] = ((lane >= (1 << s)) ? lane - (1 << s) : lane) << 2;
+        for (int s = 0; s < 6; s++) {
+            a[s] = ((lane >= (1 << s)) ? lane - (1 << s) : lane) << 2;
+            d[s] = ((lane + (1 << s) < WAVE) ? lane + (1 << s) : lane) << 2;
+        }
     }
 };
 __device__ __forceinline__ int bperm(int addr, int x) {
@@ -297,22 +311,6 @@ __device__ int find_tile(const Levels& L, int t, int v) {
     return scan_back<INCL>(L.mn0, lo, min(lo + FAN - 1, L.n[0] - 1), v);
 }
 
-template <bool INCL>
-__device__ __forceinline__ int find_in_tile(const int* __restrict__ in,
-                                            const int* __restrict__ cmin, int s, int v) {
-    static_assert(CPT == 64 && CHUNK == 64, "one 64-block per level");
-    const int c = last_in_64<INCL>(cmin, s * CPT, v);  // global chunk index
-    return last_in_64<INCL>(in, c * CHUNK, v);
-}
-
-template <bool INCL>
-__device__ int search_before_tile(const int* __restrict__ in, const Levels& L,
-                                  const int* __restrict__ cmin, int t, int v) {
-    const int s = find_tile<INCL>(L, t, v);
-    if (s < 0) return -1;
-    return find_in_tile<INCL>(in, cmin, s, v);
-}
-
 // Rightmost j in [0, K) with qual(lane lt's value j, v), or -1.
 template <bool INCL>
 __device__ __forceinline__ int last_in_lane(const TileSmem& sm, int lt, int v) {
@@ -413,14 +411,23 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     }
     sm.incl[tid] = x;
     const int wexcl = bperm(up.a[0], x);  // valid for lane > 0
+    // min of the lanes after this one inside the wave
+    int y = lmin;
+#pragma unroll
+    for (int s = 0; s < 6; s++) y = min(y, bperm(up.d[s], y));
+    // (cross-lane reads run with every lane active; select afterwards)
+    const int ynext = bperm(up.d[0], y);
+    const int wafter = (lane < WAVE - 1) ? ynext : INF;
     if (lane == WAVE - 1) sm.wmin[wave] = x;
     if ((lane & (LPC - 1)) == LPC - 1) to.cmin[(size_t)t * CPT + (tid >> LOG_LPC)] = chunk_min;
     __syncthreads();
 
-    int cross = INF;
+    int cross = INF, after = INF;
 #pragma unroll
-    for (int w = 0; w < NWAVE - 1; w++)
+    for (int w = 0; w < NWAVE; w++) {
         if (w < wave) cross = min(cross, sm.wmin[w]);
+        if (w > wave) after = min(after, sm.wmin[w]);
+    }
     // exclusive prefix-min of the tile's earlier lanes; meaningful only
     // when has_prev (INCL needs the distinction: INF is a legal key)
     const bool has_prev = tid > 0;
@@ -437,6 +444,21 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     }
     qmask &= vmask;
     omask &= vmask;
+
+    // right spine: strict suffix minima of the tile
+    unsigned smask = 0;
+    {
+        int sfx = min(wafter, after);
+#pragma unroll
+        for (int i = K - 1; i >= 0; i--) {
+            smask |= (v[i] < sfx ? 1u : 0u) << i;
+            sfx = min(sfx, v[i]);
+        }
+        // the tile's last element has nothing after it: always on the spine
+        // (INF stands for "nothing" above, but INF is also a legal key)
+        if (tid == NT - 1) smask |= 1u << (K - 1);
+        smask &= vmask;
+    }
 
     // 3a. the lane's first element (always unanswered in-lane) against the
     //     previous lane's values
@@ -472,6 +494,16 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     const int qoff = (packed & 0xFFFF) - cq, qtotal = ptotal & 0xFFFF;
     const int ooff = (packed >> 16) - co;
     if (lane == WAVE - 1) sm.wopen[wave] = ptotal >> 16;
+    // spine entries after this lane inside the wave (suffix count)
+    const int cs = __builtin_popcount(smask);
+    int zs = cs;
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+        const int z = bperm(up.d[s], zs);
+        zs += (lane + (1 << s) < WAVE) ? z : 0;
+    }
+    if (lane == 0) sm.wspine[wave] = zs;
+    const int safter_w = zs - cs;
 
     // 3b. the rest through the wave's queue: compact, resolve one query per
     //     lane, hand the answers back; the owner overwrites its placeholder
@@ -519,15 +551,31 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
                 obase++;
             }
     }
+    // 5. right spine: ranks from the tile's end, the first RS recorded
+    int srank = safter_w;
+#pragma unroll
+    for (int w = 1; w < NWAVE; w++)
+        if (w > wave) srank += sm.wspine[w];
+    if (smask && srank < RS) {
+#pragma unroll
+        for (int i = K - 1; i >= 0; i--)
+            if (smask & (1u << i)) {
+                if (srank < RS)
+                    to.spine[(size_t)t * RS + srank] = make_int2(tid * K + i, v[i]);
+                srank++;
+            }
+    }
     if (tid == 0) {
-        int m = sm.wmin[0], total = sm.wopen[0];
+        int m = sm.wmin[0], total = sm.wopen[0], stotal = sm.wspine[0];
 #pragma unroll
         for (int w = 1; w < NWAVE; w++) {
             m = min(m, sm.wmin[w]);
             total += sm.wopen[w];
+            stotal += sm.wspine[w];
         }
         L.mn0[t] = m;
         to.qcount[t] = total;
+        to.scount[t] = stotal;
         if (L.nlev > 1) atomicMin(&L.key[1][t >> LOG_FAN], make_key(L.epoch, m));
     }
     __syncthreads();  // sm is reused by the next tile
@@ -561,31 +609,115 @@ __device__ __forceinline__ void build_upper_levels(const Levels& L) {
     }
 }
 
+// Wave-cooperative version of find_tile for one query value v (uniform):
+// every level step is one coalesced 64-entry load and a ballot, so an answer
+// arbitrarily far back costs a few rounds instead of a per-lane walk.
+__device__ __forceinline__ int level_entry(const Levels& L, int lvl, int idx) {
+    return lvl == 0 ? L.mn0[idx] : key_value(L.key[lvl][idx], L.epoch);
+}
+
+template <bool INCL>
+__device__ int find_tile_wave(const Levels& L, int tb, int v) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    int x = tb, lvl = 0, found = -1;  // x: exclusive bound at level lvl
+    while (found < 0) {
+        if (x <= 0) return -1;  // unreachable when guaranteed
+        const int gs = ((x - 1) >> LOG_FAN) << LOG_FAN;
+        const int idx = gs + lane;
+        const bool ok = idx < x && qual<INCL>(level_entry(L, lvl, idx), v);
+        const u64 b = __ballot(ok);
+        if (b) {
+            found = gs + 63 - __clzll(b);
+        } else if (lvl == 1 && !L.upper) {
+            x = gs;  // no level 2: walk level 1 backwards group by group
+        } else {
+            x = (x - 1) >> LOG_FAN;
+            if (++lvl >= L.nlev) return -1;  // unreachable when guaranteed
+        }
+    }
+    while (lvl > 0) {
+        lvl--;
+        const int gs = found << LOG_FAN;
+        const int idx = gs + lane;
+        const bool ok = idx < L.n[lvl] && qual<INCL>(level_entry(L, lvl, idx), v);
+        const u64 b = __ballot(ok);
+        found = b ? gs + 63 - __clzll(b) : -1;
+        if (found < 0) return -1;  // unreachable: the parent's min qualifies
+    }
+    return found;
+}
+
 // ---------------------------------------------------------------------------
 // Fix-up of tile t by one wave: answer the open elements whose value beats
 // the exclusive prefix-min of all earlier tiles.  One batch of independent
 // loads brings the prefix inputs, the open list, the 64 tile mins before t
-// and tile t-1's 64 chunk mins (the latter two into the wave's LDS window);
-// the common case (answer in tile t-1) then needs one round of element
-// loads.  `win` is this wave's 128-int LDS area.
+// and the recorded right spines of tiles t-1 and t-2 (into the wave's LDS
+// window); a query answered in t-1 or t-2 then needs no further global
+// round, an older tile one more (its spine), and only a spine longer than
+// RS falls back to chunk mins + elements.
 // ---------------------------------------------------------------------------
+constexpr int FIX_LDS_INTS = WAVE + 4 * RS;  // tile-min window + 2 spines
+
+template <bool INCL>
+__device__ __forceinline__ int spine_answer_lds(const int2* sp, int cnt, int v) {
+    // lowest rank whose value qualifies (values decrease with rank)
+    const int m = min(cnt, RS);
+    for (int r = 0; r < m; r++)
+        if (qual<INCL>(sp[r].y, v)) return sp[r].x;
+    return -1;
+}
+
+template <bool INCL>
+__device__ __forceinline__ int answer_in_tile(const int* __restrict__ in, const TileOut& to,
+                                              int s, int v, const int2* sp_lds, int cnt) {
+    int pos = -1;
+    const int m = min(cnt, RS);
+    if (sp_lds) {
+        pos = spine_answer_lds<INCL>(sp_lds, cnt, v);
+    } else {
+        // two batches of 16 ranks (8 dwordx4 each), lowest rank wins
+        const int4* p = reinterpret_cast<const int4*>(to.spine + (size_t)s * RS);
+#pragma unroll
+        for (int h = 0; h < 2 && pos < 0; h++) {
+            int4 w[RS / 4];
+#pragma unroll
+            for (int k = 0; k < RS / 4; k++) w[k] = p[h * (RS / 4) + k];
+#pragma unroll
+            for (int k = RS / 4 - 1; k >= 0; k--) {
+                const int r = h * (RS / 2) + 2 * k;
+                pos = (r + 1 < m && qual<INCL>(w[k].w, v)) ? w[k].z : pos;
+                pos = (r < m && qual<INCL>(w[k].y, v)) ? w[k].x : pos;
+            }
+        }
+    }
+    if (pos >= 0) return s * TILE + pos;
+    // deeper than the recorded spine: chunk mins, then the chunk
+    int c = last_in_32<INCL>(to.cmin, s * CPT + 32, v);
+    if (c < 0) c = last_in_32<INCL>(to.cmin, s * CPT, v);
+    if (c < 0) return -1;  // unreachable: the tile's min qualifies
+    const int e = last_in_32<INCL>(in, c * CHUNK + 32, v);
+    return e >= 0 ? e : last_in_32<INCL>(in, c * CHUNK, v);
+}
+
 template <bool INCL>
 __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __restrict__ out,
-                                         int n, int t, const Levels& L,
-                                         const int* __restrict__ cmin,
-                                         const int* __restrict__ qcount,
-                                         const int2* __restrict__ qlist, int* win) {
-    static_assert(CPT == WAVE, "one chunk min of tile t-1 per lane");
+                                         int n, int t, const Levels& L, const TileOut& to,
+                                         int* lds) {
     if (t == 0) return;
     const int lane = threadIdx.x & (WAVE - 1);
     const int tile_start = t * TILE;
     const int tw = t - WAVE + lane;  // tile whose min this lane holds
     const int mw = tw >= 0 ? L.mn0[tw] : INF;
-    const int cw = cmin[(size_t)(t - 1) * CPT + lane];
-    const int cnt = qcount[t];
+    // spines of t-1 (lanes 0..RS-1) and t-2 (lanes RS..2RS-1)
+    const int st = (lane < RS) ? t - 1 : t - 2;
+    int2 sw = make_int2(-1, INF);
+    if (lane < 2 * RS && st >= 0) sw = to.spine[(size_t)st * RS + (lane & (RS - 1))];
+    const int sc1 = to.scount[t - 1];
+    const int sc2 = (t >= 2) ? to.scount[t - 2] : 0;
+    const int cnt = to.qcount[t];
     const int v1 = in[tile_start];
     int2 e = make_int2(0, INF);
-    if (lane < min(cnt, QS)) e = qlist[(size_t)t * QS + lane];
+    if (lane < min(cnt, QS)) e = to.qlist[(size_t)t * QS + lane];
     // exclusive prefix-min of tile t: level 0 from the window, upper levels
     // from the keys
     int p = (tw >= ((t >> LOG_FAN) << LOG_FAN)) ? mw : INF;
@@ -608,13 +740,17 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     }
     const int P = wave_min(p);
     if (!qual<INCL>(P, v1)) return;  // every open element answers -1
+    int* win = lds;
+    int2* sp = reinterpret_cast<int2*>(lds + WAVE);
     win[lane] = mw;
-    win[WAVE + lane] = cw;
+    if (lane < 2 * RS) sp[lane] = sw;
     wave_lds_sync();
     if (cnt <= QS) {
-        if (lane < cnt && qual<INCL>(P, e.y)) {
-            const int v = e.y;
-            // tile: the window first (tiles t-64 .. t-1), then the hierarchy
+        const bool active = lane < cnt && qual<INCL>(P, e.y);
+        const int v = e.y;
+        // tile: the window first (tiles t-64 .. t-1) ...
+        int ts = -1;
+        if (active) {
             int s = -1;
             for (int i = WAVE - 4; i >= 0 && s < 0; i -= 4) {
                 const int4 w = *reinterpret_cast<const int4*>(&win[i]);
@@ -623,21 +759,24 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
             }
             // window entries of tiles < 0 hold INF, which never qualifies
             // except under INCL with v == INF: exclude them explicitly
-            if (s >= 0 && t - WAVE + s < 0) s = -1;
-            int c;
-            if (s == WAVE - 1) {
-                c = -1;
-                for (int i = WAVE - 4; i >= 0 && c < 0; i -= 4) {
-                    const int4 w = *reinterpret_cast<const int4*>(&win[WAVE + i]);
-                    c = qual<INCL>(w.w, v) ? i + 3 : qual<INCL>(w.z, v) ? i + 2
-                      : qual<INCL>(w.y, v) ? i + 1 : qual<INCL>(w.x, v) ? i : -1;
-                }
-                c += (t - 1) * CPT;
-            } else {
-                const int st = (s >= 0) ? t - WAVE + s : find_tile<INCL>(L, t - WAVE, v);
-                c = last_in_64<INCL>(cmin, st * CPT, v);
-            }
-            out[tile_start + e.x] = last_in_64<INCL>(in, c * CHUNK, v);
+            if (s >= 0 && t - WAVE + s >= 0) ts = t - WAVE + s;
+        }
+        // ... then, for the rare lanes whose tile is older, the whole wave
+        // walks the hierarchy for one of them at a time
+        u64 deep = __ballot(active && ts < 0);
+        while (deep) {
+            const int l = __ffsll(deep) - 1;
+            deep &= deep - 1;
+            const int r = find_tile_wave<INCL>(L, t - WAVE, __shfl(v, l, WAVE));
+            if (lane == l) ts = r;
+        }
+        if (active) {
+            int r;
+            if (ts < 0) r = -1;  // unreachable: P qualifies, so some tile does
+            else if (ts == t - 1) r = answer_in_tile<INCL>(in, to, ts, v, sp, sc1);
+            else if (ts == t - 2) r = answer_in_tile<INCL>(in, to, ts, v, sp + RS, sc2);
+            else r = answer_in_tile<INCL>(in, to, ts, v, nullptr, to.scount[ts]);
+            out[tile_start + e.x] = r;
         }
         return;
     }
@@ -656,8 +795,10 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
         const bool ex_real = lane > 0 || have;
         ex = (lane == 0) ? rm : min(ex, rm);
         const bool open = valid && !(ex_real && qual<INCL>(ex, xv));
-        if (open && qual<INCL>(P, xv))
-            out[idx] = search_before_tile<INCL>(in, L, cmin, t, xv);
+        if (open && qual<INCL>(P, xv)) {
+            const int ts = find_tile<INCL>(L, t, xv);
+            out[idx] = (ts < 0) ? -1 : answer_in_tile<INCL>(in, to, ts, xv, nullptr, to.scount[ts]);
+        }
         rm = min(rm, __shfl(inc, WAVE - 1, WAVE));
         have = true;
         if (!qual<INCL>(P, rm)) break;  // later open elements are <= rm

@@ -23,6 +23,8 @@ struct apsep_scratch {
     int* cmin;
     int* qcount;
     int2* qlist;
+    int* scount;
+    int2* spine;
     int coop_grid;        // SPT persistent grid size (occupancy x CUs)
     apsep::GridBarrier* bar;
 };

@@ -37,17 +37,17 @@ __global__ __launch_bounds__(NT) void k_tile_pass(const int* __restrict__ in,
 __global__ __launch_bounds__(256) void k_upper_levels(Levels L) { build_upper_levels(L); }
 
 // WSTL pass 3: one wave per tile.
+// (1024-lane workgroups: 16 tiles per workgroup, fewer dispatches)
+constexpr int FIX_WG = 1024;
+
 template <bool INCL>
-__global__ __launch_bounds__(256) void k_fix_pass(const int* __restrict__ in,
-                                                  int* __restrict__ out, int n,
-                                                  Levels L,
-                                                  const int* __restrict__ cmin,
-                                                  const int* __restrict__ qcount,
-                                                  const int2* __restrict__ qlist) {
-    __shared__ int win[256 / WAVE][2 * WAVE];
-    const int t = (int)(blockIdx.x * (blockDim.x / WAVE) + (threadIdx.x >> 6));
+__global__ __launch_bounds__(FIX_WG) void k_fix_pass(const int* __restrict__ in,
+                                                     int* __restrict__ out, int n,
+                                                     Levels L, TileOut to) {
+    __shared__ int lds[FIX_WG / WAVE][FIX_LDS_INTS];
+    const int t = (int)(blockIdx.x * (FIX_WG / WAVE) + (threadIdx.x >> 6));
     if (t >= L.nt) return;
-    fix_tile<INCL>(in, out, n, t, L, cmin, qcount, qlist, win[threadIdx.x >> 6]);
+    fix_tile<INCL>(in, out, n, t, L, to, lds[threadIdx.x >> 6]);
 }
 
 // SPT: one persistent launch of fully co-resident workgroups (launched
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SPT_WAVES_PE
 void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
            TileOut to, Levels L, GridBarrier* bar) {
     __shared__ TileSmem sm;
-    __shared__ int win[NWAVE][2 * WAVE];
+    __shared__ int lds[NWAVE][FIX_LDS_INTS];
     unsigned gen = grid_barrier_start(bar);
     const int nt = L.nt;
     const UpAddr up;
@@ -76,7 +76,7 @@ void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
     const int gw = (int)(blockIdx.x * NWAVE + (threadIdx.x >> 6));
     const int nw = (int)(gridDim.x * NWAVE);
     for (int t = gw; t < nt; t += nw)
-        fix_tile<INCL>(in, out, n, t, L, to.cmin, to.qcount, to.qlist, win[threadIdx.x >> 6]);
+        fix_tile<INCL>(in, out, n, t, L, to, lds[threadIdx.x >> 6]);
 }
 
 // ---------------------------------------------------------------------------
@@ -105,8 +105,10 @@ int64_t single_scratch_bytes(int64_t n_cap) {
     b += (round4((int64_t)nt * CPT) + 4) * 4;       // cmin
     b += (round4(nt) + 4) * 4;                        // qcount
     b += ((int64_t)nt * QS + 2) * 8;                  // qlist
+    b += (round4(nt) + 4) * 4;                        // scount
+    b += ((int64_t)nt * RS + 2) * 8;                  // spine
     b += (int64_t)sizeof(GridBarrier);                // grid barrier
-    return b + 256 * 10;                              // alignment slack
+    return b + 256 * 12;                              // alignment slack
 }
 
 static void* carve(char*& p, int64_t bytes) {
@@ -134,6 +136,8 @@ apsep_status single_scratch_init(apsep_scratch* s) {
     s->cmin = (int*)carve(p, (round4((int64_t)nt * CPT) + 4) * 4);
     s->qcount = (int*)carve(p, (round4(nt) + 4) * 4);
     s->qlist = (int2*)carve(p, ((int64_t)nt * QS + 2) * 8);
+    s->scount = (int*)carve(p, (round4(nt) + 4) * 4);
+    s->spine = (int2*)carve(p, ((int64_t)nt * RS + 2) * 8);
     s->bar = (GridBarrier*)carve(p, (int64_t)sizeof(GridBarrier));
     if ((char*)p > (char*)s->base + s->bytes) return APSEP_ERR_ALLOC;  // sizing bug guard
     if (hipMemset(s->base, 0, (size_t)s->bytes) != hipSuccess) {
@@ -201,14 +205,15 @@ static apsep_status run_wstl(apsep_scratch* s, const int* in, int* out, int n,
     const int nt = (n + TILE - 1) / TILE;
     Levels L;
     if (fill_levels(s, nt, L, st) != APSEP_OK) return APSEP_ERR_LAUNCH;
-    TileOut to{s->cmin, s->qcount, s->qlist};
+    TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
     hipLaunchKernelGGL(k_tile_pass<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L);
     if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
     if (nt <= 1) return APSEP_OK;
     if (L.upper)
         hipLaunchKernelGGL(k_upper_levels, dim3(1), dim3(256), 0, st, L);
-    hipLaunchKernelGGL(k_fix_pass<INCL>, dim3((nt + 3) / 4), dim3(256), 0, st, in, out,
-                       n, L, s->cmin, s->qcount, s->qlist);
+    constexpr int tpw = FIX_WG / WAVE;
+    hipLaunchKernelGGL(k_fix_pass<INCL>, dim3((nt + tpw - 1) / tpw), dim3(FIX_WG), 0, st, in,
+                       out, n, L, to);
     if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
     return APSEP_OK;
 }
@@ -219,7 +224,7 @@ static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
     const int nt = (n + TILE - 1) / TILE;
     Levels L;
     if (fill_levels(s, nt, L, st) != APSEP_OK) return APSEP_ERR_LAUNCH;
-    TileOut to{s->cmin, s->qcount, s->qlist};
+    TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
     int grid = s->coop_grid < nt ? s->coop_grid : nt;
     if (grid < 1) grid = 1;
     GridBarrier* bar = s->bar;
