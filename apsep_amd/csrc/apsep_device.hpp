@@ -115,11 +115,14 @@ struct TileOut {
 
 // LDS of one tile.  Element e = lane*K + i lives in val4[i >> 2][lane],
 // component i & 3: lane-contiguous dwordx4 rows, so every ds_*_b128 of the
-// tile pass is bank-conflict free.
+// tile pass is bank-conflict free.  Lane-min windows live inside 16-lane
+// rows (the reach of a DPP row shift): win[s][l] = min of the lane mins of
+// lanes max(l - 2^s + 1, row start) .. l for s = 0..3, win[4][l] = min of
+// row start .. l.
 struct TileSmem {
     int4 val4[K4][NT];          // the tile's values
-    int win[6][NT];             // window mins of lane mins, sizes 1..32
-    int incl[NT];               // inclusive prefix-min of lane mins in the wave
+    int win[5][NT];             // lane-min windows inside rows
+    int4 rmin[NWAVE];           // row mins of each wave
     int wmin[NWAVE];            // wave mins
     int wopen[NWAVE];           // open elements per wave
     int wspine[NWAVE];          // right-spine entries per wave
@@ -136,24 +139,21 @@ __device__ __forceinline__ int wave_shfl_up(int x, int d) {
     return __shfl_up(x, d, WAVE);
 }
 
-// ds_bpermute byte addresses of lane - 2^s (own lane below 2^s), computed
-// once per thread: every shift-up of the tile pass is then one LDS-crossbar
-// op plus the combining VALU op.
-struct UpAddr {
-    int a[6];   // lane - 2^s (own lane below)
-    int d[6];   // lane + 2^s (own lane above 63 - 2^s)
-    __device__ __forceinline__ UpAddr() {
-        const int lane = threadIdx.x & (WAVE - 1);
-#pragma unroll
-        for (int s = 0; s < 6; s++) {
-            a[s] = ((lane >= (1 << s)) ? lane - (1 << s) : lane) << 2;
-            d[s] = ((lane + (1 << s) < WAVE) ? lane + (1 << s) : lane) << 2;
-        }
-    }
-};
-__device__ __forceinline__ int bperm(int addr, int x) {
-    return __builtin_amdgcn_ds_bpermute(addr, x);
+// DPP data-parallel-primitive controls (GFX9 encoding): a VALU operand read
+// from another lane of the same row (16 lanes) or wave, at no extra latency.
+constexpr int DPP_ROW_SHL = 0x100;      // + n: lane + n inside the row
+constexpr int DPP_ROW_SHR = 0x110;      // + n: lane - n inside the row
+constexpr int DPP_WAVE_SHR1 = 0x138;    // lane - 1 across the wave
+constexpr int DPP_ROW_BCAST15 = 0x142;  // lane 15 of each row, read by the next row
+constexpr int DPP_ROW_BCAST31 = 0x143;  // lane 31, read by rows 2 and 3
+
+// The value of x at the lane CTRL names; `ident` where that lane lies outside
+// the row / wave or the reading row is not in ROWS.
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp(int ident, int x) {
+    return __builtin_amdgcn_update_dpp(ident, x, CTRL, ROWS, 0xf, false);
 }
+__device__ __forceinline__ int readlane(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
 __device__ __forceinline__ int lane63(int x) { return __builtin_amdgcn_readlane(x, WAVE - 1); }
 
 // Inclusive min scan over the wave (Hillis-Steele), lanes below the shift
@@ -327,26 +327,38 @@ __device__ __forceinline__ int last_in_lane(const TileSmem& sm, int lt, int v) {
 }
 
 // Answer (tile-relative element index) of the query value `val` asked by
-// tile lane qt, given that some lane before qt holds a qualifying element.
+// tile lane qt, given that some lane before qt holds a qualifying element:
+// pick the nearest row holding one (own row, earlier rows of the wave, then
+// the nearest qualifying earlier wave's last qualifying row), lift inside
+// that row over the windows, then scan the lane.
 template <bool INCL>
 __device__ __forceinline__ int intile_answer(const TileSmem& sm, int qt, int val) {
-    const int wq = qt >> 6, ql = qt & (WAVE - 1);
-    int wb, p;
-    if (ql > 0 && qual<INCL>(sm.incl[qt - 1], val)) {
-        wb = wq * WAVE;
-        p = ql - 1;
+    const int wq = qt >> 6, ql = qt & (WAVE - 1), r = ql >> 4;
+    int p;
+    if ((ql & 15) > 0 && qual<INCL>(sm.win[4][qt - 1], val)) {
+        p = qt - 1;
     } else {
-        int w = wq - 1;  // an earlier wave qualifies; bounded anyway
-        while (w > 0 && !qual<INCL>(sm.wmin[w], val)) w--;
-        wb = w * WAVE;
-        p = WAVE - 1;
+        int4 rm = sm.rmin[wq];
+        int rr = (r > 2 && qual<INCL>(rm.z, val)) ? 2
+               : (r > 1 && qual<INCL>(rm.y, val)) ? 1
+               : (r > 0 && qual<INCL>(rm.x, val)) ? 0 : -1;
+        int w = wq;
+        if (rr < 0) {
+            w = wq - 1;  // an earlier wave qualifies; bounded anyway
+            while (w > 0 && !qual<INCL>(sm.wmin[w], val)) w--;
+            rm = sm.rmin[w];
+            rr = qual<INCL>(rm.w, val) ? 3 : qual<INCL>(rm.z, val) ? 2
+               : qual<INCL>(rm.y, val) ? 1 : 0;
+        }
+        p = w * WAVE + rr * 16 + 15;
     }
+    // nearest lane <= p of p's row whose min qualifies (one exists)
 #pragma unroll
-    for (int s = 5; s >= 0; s--)
-        if (!qual<INCL>(sm.win[s][wb + p], val)) p -= 1 << s;
-    const int lt = wb + p;
-    const int j = last_in_lane<INCL>(sm, lt, val);
-    return lt * K + (j < 0 ? 0 : j);
+    for (int s = 3; s >= 0; s--)
+        if (!qual<INCL>(sm.win[s][p], val)) p -= 1 << s;
+    p = max(p, 0);
+    const int j = last_in_lane<INCL>(sm, p, val);
+    return p * K + (j < 0 ? 0 : j);
 }
 
 // ---------------------------------------------------------------------------
@@ -372,11 +384,11 @@ __device__ __forceinline__ void load_lane(const int* __restrict__ in, int n, int
 
 template <bool INCL>
 __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, const int* v,
-                                           TileSmem& sm, const TileOut& to,
-                                           const Levels& L, const UpAddr& up) {
+                                           TileSmem& sm, const TileOut& to, const Levels& L) {
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     const int wave = tid >> 6;
+    const int row = lane >> 4;
     const int tile_start = t * TILE;
     const int base = tile_start + tid * K;
     // elements of this lane inside the array (the last tile is partial; its
@@ -400,26 +412,34 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     for (int q = 0; q < K4; q++)
         sm.val4[q][tid] = make_int4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
 
-    // 2. window mins of lane mins (Hillis-Steele within the wave)
-    int x = lmin, chunk_min = 0;
-    sm.win[0][tid] = x;
-#pragma unroll
-    for (int s = 1; s <= 6; s++) {
-        x = min(x, bperm(up.a[s - 1], x));
-        if (s < 6) sm.win[s][tid] = x;
-        if (s == LOG_LPC) chunk_min = x;  // window of LPC lanes = 64 elements
+    // 2. lane-min windows inside rows, wave prefix / suffix mins (DPP)
+    const int w2 = min(lmin, dpp<DPP_ROW_SHR + 1>(INF, lmin));
+    const int w4 = min(w2, dpp<DPP_ROW_SHR + 2>(INF, w2));
+    const int w8 = min(w4, dpp<DPP_ROW_SHR + 4>(INF, w4));
+    const int w16 = min(w8, dpp<DPP_ROW_SHR + 8>(INF, w8));  // row start .. lane
+    sm.win[0][tid] = lmin;
+    sm.win[1][tid] = w2;
+    sm.win[2][tid] = w4;
+    sm.win[3][tid] = w8;
+    sm.win[4][tid] = w16;
+    // window of LPC lanes = one 64-element chunk
+    if ((lane & (LPC - 1)) == LPC - 1) to.cmin[(size_t)t * CPT + (tid >> LOG_LPC)] = w4;
+    int winc = min(w16, dpp<DPP_ROW_BCAST15, 0xa>(INF, w16));
+    winc = min(winc, dpp<DPP_ROW_BCAST31, 0xc>(INF, winc));  // wave start .. lane
+    const int wexcl = dpp<DPP_WAVE_SHR1>(INF, winc);          // wave start .. lane - 1
+    const int rm0 = readlane(w16, 15), rm1 = readlane(w16, 31);
+    const int rm2 = readlane(w16, 47), rm3 = readlane(w16, 63);
+    const int r23 = min(rm2, rm3), r123 = min(rm1, r23);
+    int sfx = min(lmin, dpp<DPP_ROW_SHL + 1>(INF, lmin));
+    sfx = min(sfx, dpp<DPP_ROW_SHL + 2>(INF, sfx));
+    sfx = min(sfx, dpp<DPP_ROW_SHL + 4>(INF, sfx));
+    sfx = min(sfx, dpp<DPP_ROW_SHL + 8>(INF, sfx));           // lane .. row end
+    const int later_rows = row == 0 ? r123 : row == 1 ? r23 : row == 2 ? rm3 : INF;
+    const int wafter = min(dpp<DPP_ROW_SHL + 1>(INF, sfx), later_rows);  // lane + 1 .. wave end
+    if (lane == 0) {
+        sm.rmin[wave] = make_int4(rm0, rm1, rm2, rm3);
+        sm.wmin[wave] = min(rm0, r123);
     }
-    sm.incl[tid] = x;
-    const int wexcl = bperm(up.a[0], x);  // valid for lane > 0
-    // min of the lanes after this one inside the wave
-    int y = lmin;
-#pragma unroll
-    for (int s = 0; s < 6; s++) y = min(y, bperm(up.d[s], y));
-    // (cross-lane reads run with every lane active; select afterwards)
-    const int ynext = bperm(up.d[0], y);
-    const int wafter = (lane < WAVE - 1) ? ynext : INF;
-    if (lane == WAVE - 1) sm.wmin[wave] = x;
-    if ((lane & (LPC - 1)) == LPC - 1) to.cmin[(size_t)t * CPT + (tid >> LOG_LPC)] = chunk_min;
     __syncthreads();
 
     int cross = INF, after = INF;
@@ -431,32 +451,44 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     // exclusive prefix-min of the tile's earlier lanes; meaningful only
     // when has_prev (INCL needs the distinction: INF is a legal key)
     const bool has_prev = tid > 0;
-    const int pmx = (lane > 0) ? min(wexcl, cross) : cross;
+    const int pmx = min(wexcl, cross);
 
-    // classify lane-unanswered elements (branch-free)
-    unsigned qmask = 0, omask = 0;
+    // classify lane-unanswered elements.  They are the lane's prefix minima
+    // (non-increasing, the last one is the lane min), so when pmx beats the
+    // lane min it beats them all and every one has an answer in the tile;
+    // only lanes holding tile prefix minima need the per-element split.
+    unsigned rec = 0;
 #pragma unroll
-    for (int i = 0; i < K; i++) {
-        const bool rec = res[i] < 0;
-        const bool in_tile = rec && has_prev && qual<INCL>(pmx, v[i]);
-        qmask |= (in_tile ? 1u : 0u) << i;
-        omask |= ((rec && !in_tile) ? 1u : 0u) << i;
+    for (int i = 0; i < K; i++) rec |= ((unsigned)res[i] >> 31) << i;
+    unsigned qmask = rec, omask = 0;
+    if (!has_prev || !qual<INCL>(pmx, lmin)) {
+        unsigned beaten = 0;
+        if (has_prev) {
+#pragma unroll
+            for (int i = 0; i < K; i++) beaten |= (qual<INCL>(pmx, v[i]) ? 1u : 0u) << i;
+        }
+        qmask = rec & beaten;
+        omask = rec & ~beaten;
     }
     qmask &= vmask;
     omask &= vmask;
 
-    // right spine: strict suffix minima of the tile
+    // right spine: strict suffix minima of the tile.  A lane holds some iff
+    // its min is below everything after it; the tile's last element has
+    // nothing after it and is always on the spine (INF stands for "nothing"
+    // here, but INF is also a legal key).
     unsigned smask = 0;
     {
-        int sfx = min(wafter, after);
+        const int A = min(wafter, after);
+        if (lmin < A || tid == NT - 1) {
+            int m = A;
 #pragma unroll
-        for (int i = K - 1; i >= 0; i--) {
-            smask |= (v[i] < sfx ? 1u : 0u) << i;
-            sfx = min(sfx, v[i]);
+            for (int i = K - 1; i >= 0; i--) {
+                smask |= (v[i] < m ? 1u : 0u) << i;
+                m = min(m, v[i]);
+            }
+            if (tid == NT - 1) smask |= 1u << (K - 1);
         }
-        // the tile's last element has nothing after it: always on the spine
-        // (INF stands for "nothing" above, but INF is also a legal key)
-        if (tid == NT - 1) smask |= 1u << (K - 1);
         smask &= vmask;
     }
 
@@ -485,24 +517,29 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     // one scan for both counts: queries (low 16 bits), open (high 16 bits)
     const int cq = __builtin_popcount(qmask), co = __builtin_popcount(omask);
     int packed = cq | (co << 16);
-#pragma unroll
-    for (int s = 0; s < 6; s++) {
-        const int y = bperm(up.a[s], packed);
-        packed += (lane >= (1 << s)) ? y : 0;
-    }
-    const int ptotal = lane63(packed);
+    packed += dpp<DPP_ROW_SHR + 1>(0, packed);
+    packed += dpp<DPP_ROW_SHR + 2>(0, packed);
+    packed += dpp<DPP_ROW_SHR + 4>(0, packed);
+    packed += dpp<DPP_ROW_SHR + 8>(0, packed);
+    packed += dpp<DPP_ROW_BCAST15, 0xa>(0, packed);
+    packed += dpp<DPP_ROW_BCAST31, 0xc>(0, packed);
+    const int ptotal = readlane(packed, WAVE - 1);
     const int qoff = (packed & 0xFFFF) - cq, qtotal = ptotal & 0xFFFF;
     const int ooff = (packed >> 16) - co;
-    if (lane == WAVE - 1) sm.wopen[wave] = ptotal >> 16;
     // spine entries after this lane inside the wave (suffix count)
     const int cs = __builtin_popcount(smask);
     int zs = cs;
-#pragma unroll
-    for (int s = 0; s < 6; s++) {
-        const int z = bperm(up.d[s], zs);
-        zs += (lane + (1 << s) < WAVE) ? z : 0;
+    zs += dpp<DPP_ROW_SHL + 1>(0, zs);
+    zs += dpp<DPP_ROW_SHL + 2>(0, zs);
+    zs += dpp<DPP_ROW_SHL + 4>(0, zs);
+    zs += dpp<DPP_ROW_SHL + 8>(0, zs);  // lane .. row end
+    const int rt0 = readlane(zs, 0), rt1 = readlane(zs, 16);
+    const int rt2 = readlane(zs, 32), rt3 = readlane(zs, 48);
+    zs += row == 0 ? rt1 + rt2 + rt3 : row == 1 ? rt2 + rt3 : row == 2 ? rt3 : 0;
+    if (lane == 0) {
+        sm.wopen[wave] = ptotal >> 16;
+        sm.wspine[wave] = rt0 + rt1 + rt2 + rt3;
     }
-    if (lane == 0) sm.wspine[wave] = zs;
     const int safter_w = zs - cs;
 
     // 3b. the rest through the wave's queue: compact, resolve one query per
@@ -585,10 +622,10 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
 template <bool INCL>
 __device__ __forceinline__ void tile_step(const int* __restrict__ in, int* __restrict__ out,
                                           int n, int t, TileSmem& sm, const TileOut& to,
-                                          const Levels& L, const UpAddr& up) {
+                                          const Levels& L) {
     int v[K];
     load_lane(in, n, t, v);
-    solve_tile<INCL>(out, n, t, v, sm, to, L, up);
+    solve_tile<INCL>(out, n, t, v, sm, to, L);
 }
 
 // ---------------------------------------------------------------------------

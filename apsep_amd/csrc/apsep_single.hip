@@ -29,8 +29,7 @@ __global__ __launch_bounds__(NT) void k_tile_pass(const int* __restrict__ in,
                                                   int* __restrict__ out, int n,
                                                   TileOut to, Levels L) {
     __shared__ TileSmem sm;
-    const UpAddr up;
-    tile_step<INCL>(in, out, n, (int)blockIdx.x, sm, to, L, up);
+    tile_step<INCL>(in, out, n, (int)blockIdx.x, sm, to, L);
 }
 
 // WSTL pass 2 (only when level 1 exceeds UPPER_MIN entries): levels >= 2.
@@ -64,9 +63,8 @@ void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
     __shared__ int lds[NWAVE][FIX_LDS_INTS];
     unsigned gen = grid_barrier_start(bar);
     const int nt = L.nt;
-    const UpAddr up;
     for (int t = (int)blockIdx.x; t < nt; t += (int)gridDim.x)
-        tile_step<INCL>(in, out, n, t, sm, to, L, up);
+        tile_step<INCL>(in, out, n, t, sm, to, L);
     if (nt <= 1) return;  // uniform across the grid
     if (!grid_barrier(bar, gridDim.x, gen)) return;
     if (L.upper) {
