@@ -35,7 +35,7 @@ EXPORTS = [
     "apsep_status_string", "apsep_version", "apsep_scratch_create",
     "apsep_scratch_destroy", "apsep_scratch_bytes", "apsep_run",
     "apsep_launch", "apsep_run_mode", "apsep_scratch_query",
-    "apsep_scratch_check",
+    "apsep_scratch_check", "apsep_time_kernels",
 ]
 
 _lib = None
@@ -48,6 +48,12 @@ class ScratchInfo(ctypes.Structure):
                 ("device", ctypes.c_int32), ("tile_elems", ctypes.c_int32),
                 ("tiles", ctypes.c_int32), ("levels", ctypes.c_int32),
                 ("persistent_blocks", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class KernelTimes(ctypes.Structure):
+    """apsep_kernel_times (include/apsep/apsep.h): summed device time per kernel."""
+    _fields_ = [("runs", ctypes.c_int64), ("tile_ms", ctypes.c_double),
+                ("levels_ms", ctypes.c_double), ("fix_ms", ctypes.c_double)]
 
 
 class ApsepError(RuntimeError):
@@ -77,6 +83,8 @@ def lib() -> ctypes.CDLL:
         L.apsep_run_mode.argtypes = [vp, vp, vp, i64, ci, vp]
         L.apsep_scratch_query.argtypes = [vp, ctypes.POINTER(ScratchInfo)]
         L.apsep_scratch_check.argtypes = [vp, ctypes.POINTER(i32)]
+        L.apsep_time_kernels.argtypes = [vp, vp, vp, i64, ci, ci, vp,
+                                         ctypes.POINTER(KernelTimes)]
         for f in EXPORTS:
             if f != "apsep_status_string" and f != "apsep_version":
                 getattr(L, f).restype = ci
@@ -105,7 +113,7 @@ def scratch_bytes(algo, n_cap: int, incl: bool = False) -> int:
 class Scratch:
     """Device scratch for solves of up to ``n_cap`` elements."""
 
-    def __init__(self, algo="spt", n_cap: int = 0, incl: bool = False):
+    def __init__(self, algo="auto", n_cap: int = 0, incl: bool = False):
         self._h = ctypes.c_void_p(None)
         self.n_cap = n_cap
         self.incl = bool(incl)
@@ -148,6 +156,22 @@ class Scratch:
             raise ApsepError(ERR_LAUNCH, "device-side error %d (grid barrier timeout)"
                              % e.value)
 
+    def time_kernels(self, x, out, reps: int, n: int | None = None, stream=None,
+                     incl: bool | None = None) -> KernelTimes:
+        """Launch each kernel of the solve of ``x`` into ``out`` ``reps`` times
+        back to back and return the summed device time per kernel (blocks)."""
+        import torch
+        if n is None:
+            n = x.numel()
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        t = KernelTimes()
+        _check(lib().apsep_time_kernels(self._h, x.data_ptr(), out.data_ptr(), n,
+                                        int(self.incl if incl is None else incl), reps,
+                                        stream.cuda_stream, ctypes.byref(t)),
+               "apsep_time_kernels")
+        return t
+
     def close(self):
         if self._h is not None and self._h.value:
             lib().apsep_scratch_destroy(self._h)
@@ -171,7 +195,7 @@ def launch(algo, d_in: int, d_out: int, n: int, incl: bool = False):
     _check(lib().apsep_launch(_algo(algo), d_in, d_out, n, int(incl)), "apsep_launch")
 
 
-def solve(x, algo="spt", incl: bool = False, out=None):
+def solve(x, algo="auto", incl: bool = False, out=None):
     """Convenience: allocate ``out`` (int32, like ``x``) and solve on torch's
     current stream."""
     import torch

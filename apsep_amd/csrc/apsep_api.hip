@@ -39,7 +39,9 @@ static bool valid_algo(apsep_algo a) {
 }
 
 static apsep_algo resolve_algo(apsep_algo a) {
-    if (a == APSEP_ALGO_AUTO) return APSEP_ALGO_SPT;
+    // the multi-launch solver is the fastest on gfx950: the persistent one
+    // pays for agent-scope fences at every grid barrier
+    if (a == APSEP_ALGO_AUTO) return APSEP_ALGO_WSTL;
     if (a == APSEP_ALGO_LOOKBACK) return APSEP_ALGO_SPT;  // not built yet
     return a;
 }
@@ -66,7 +68,7 @@ apsep_status apsep_scratch_create(apsep_algo algo, int64_t n_cap, int incl,
         (void)hipGetLastError();
         return APSEP_ERR_DEVICE;
     }
-    apsep_scratch* s = new (std::nothrow) apsep_scratch;
+    apsep_scratch* s = new (std::nothrow) apsep_scratch();  // zeroed
     if (!s) return APSEP_ERR_ALLOC;
     memset(s, 0, sizeof *s);
     s->algo = resolve_algo(algo);
@@ -149,6 +151,30 @@ apsep_status apsep_run_mode(apsep_scratch* s, const int32_t* d_in, int32_t* d_ou
         return APSEP_ERR_DEVICE;
     }
     return single_run(s, d_in, d_out, (int)n, incl, (hipStream_t)stream);
+}
+
+apsep_status apsep_time_kernels(apsep_scratch* s, const int32_t* d_in, int32_t* d_out,
+                                int64_t n, int incl, int reps, void* stream,
+                                apsep_kernel_times* out) {
+    if (!s || !out || reps < 1 || (incl != 0 && incl != 1)) return APSEP_ERR_INVALID_ARG;
+    memset(out, 0, sizeof *out);
+    if (n <= 0) return APSEP_OK;
+    apsep_status st = check_io(d_in, d_out, n);
+    if (st != APSEP_OK) return st;
+    if (n > s->n_cap) return APSEP_ERR_CAPACITY;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != s->device) {
+        (void)hipGetLastError();
+        return APSEP_ERR_DEVICE;
+    }
+    double ms[3] = {0, 0, 0};
+    st = single_time_kernels(s, d_in, d_out, (int)n, incl, reps, (hipStream_t)stream, ms);
+    if (st != APSEP_OK) return st;
+    out->runs = reps;
+    out->tile_ms = ms[0];
+    out->levels_ms = ms[1];
+    out->fix_ms = ms[2];
+    return APSEP_OK;
 }
 
 apsep_status apsep_launch(apsep_algo algo, const int32_t* d_in, int32_t* d_out,

@@ -37,5 +37,9 @@ apsep_status single_scratch_init(apsep_scratch* s);
 apsep_status single_run(apsep_scratch* s, const int32_t* in, int32_t* out, int n,
                         int incl, hipStream_t st);
 apsep_status single_error(apsep_scratch* s, int* code);
+// One full solve, then `reps` back-to-back launches of each of its kernels
+// (each launch is idempotent), timed per kernel batch with HIP events.
+apsep_status single_time_kernels(apsep_scratch* s, const int32_t* in, int32_t* out, int n,
+                                 int incl, int reps, hipStream_t st, double ms[3]);
 
 }  // namespace apsep

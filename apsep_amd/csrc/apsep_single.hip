@@ -206,13 +206,14 @@ static apsep_status run_wstl(apsep_scratch* s, const int* in, int* out, int n,
     TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
     hipLaunchKernelGGL(k_tile_pass<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L);
     if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
-    if (nt <= 1) return APSEP_OK;
-    if (L.upper)
+    if (nt > 1 && L.upper)
         hipLaunchKernelGGL(k_upper_levels, dim3(1), dim3(256), 0, st, L);
-    constexpr int tpw = FIX_WG / WAVE;
-    hipLaunchKernelGGL(k_fix_pass<INCL>, dim3((nt + tpw - 1) / tpw), dim3(FIX_WG), 0, st, in,
-                       out, n, L, to);
-    if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
+    if (nt > 1) {
+        constexpr int tpw = FIX_WG / WAVE;
+        hipLaunchKernelGGL(k_fix_pass<INCL>, dim3((nt + tpw - 1) / tpw), dim3(FIX_WG), 0, st,
+                           in, out, n, L, to);
+        if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
+    }
     return APSEP_OK;
 }
 
@@ -234,6 +235,69 @@ static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
         return APSEP_ERR_LAUNCH;
     }
     return APSEP_OK;
+}
+
+template <bool INCL>
+static apsep_status time_wstl(apsep_scratch* s, const int* in, int* out, int n, int reps,
+                              hipStream_t st, hipEvent_t* ev) {
+    const int nt = (n + TILE - 1) / TILE;
+    Levels L;
+    if (fill_levels(s, nt, L, st) != APSEP_OK) return APSEP_ERR_LAUNCH;
+    TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
+    constexpr int tpw = FIX_WG / WAVE;
+    // re-running a kernel on the same epoch rewrites the same values: the
+    // level-1 atomicMin sees the same keys, the fix-up the same summaries
+    (void)hipEventRecord(ev[0], st);
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_tile_pass<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L);
+    (void)hipEventRecord(ev[1], st);
+    if (nt > 1 && L.upper)
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL(k_upper_levels, dim3(1), dim3(256), 0, st, L);
+    (void)hipEventRecord(ev[2], st);
+    if (nt > 1)
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL(k_fix_pass<INCL>, dim3((nt + tpw - 1) / tpw), dim3(FIX_WG), 0,
+                               st, in, out, n, L, to);
+    (void)hipEventRecord(ev[3], st);
+    return hipGetLastError() == hipSuccess ? APSEP_OK : APSEP_ERR_LAUNCH;
+}
+
+apsep_status single_time_kernels(apsep_scratch* s, const int32_t* in, int32_t* out, int n,
+                                 int incl, int reps, hipStream_t st, double ms[3]) {
+    hipEvent_t ev[4];
+    int made = 0;
+    apsep_status r = APSEP_OK;
+    for (; made < 4; made++)
+        if (hipEventCreate(&ev[made]) != hipSuccess) {
+            (void)hipGetLastError();
+            r = APSEP_ERR_DEVICE;
+            break;
+        }
+    if (r == APSEP_OK) {
+        if (s->algo == APSEP_ALGO_WSTL) {
+            r = incl ? time_wstl<true>(s, in, out, n, reps, st, ev)
+                     : time_wstl<false>(s, in, out, n, reps, st, ev);
+        } else {
+            // the persistent kernel is the whole solve
+            (void)hipEventRecord(ev[0], st);
+            for (int k = 0; k < reps && r == APSEP_OK; k++) r = single_run(s, in, out, n, incl, st);
+            (void)hipEventRecord(ev[1], st);
+            (void)hipEventRecord(ev[2], st);
+            (void)hipEventRecord(ev[3], st);
+        }
+    }
+    if (r == APSEP_OK) {
+        if (hipEventSynchronize(ev[3]) != hipSuccess) r = APSEP_ERR_LAUNCH;
+        for (int k = 0; k < 3 && r == APSEP_OK; k++) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ev[k], ev[k + 1]) != hipSuccess) r = APSEP_ERR_LAUNCH;
+            ms[k] = t;
+        }
+        if (r != APSEP_OK) (void)hipGetLastError();
+    }
+    for (int k = 0; k < made; k++) (void)hipEventDestroy(ev[k]);
+    return r;
 }
 
 apsep_status single_run(apsep_scratch* s, const int32_t* in, int32_t* out, int n,

@@ -8,9 +8,14 @@ resident in HBM before timing starts.  Useful bytes per solve = 2 * N * 4
 
 Printed JSON line (rank 0, one line):
   value        aggregate useful GB/s over the three solves of a step
-  roofline     the solver kernel: achieved = 8 N / (mean kernel time),
-               peak = 8000 GB/s HBM; traffic = HBM bytes per launch from a
-               rocprofv3 FETCH_SIZE/WRITE_SIZE run (null when not measured)
+  roofline     the dominant kernel (the tile pass): achieved = 8 N / (its
+               mean device time: after the timed steps, each kernel of the
+               solve is launched `steps` times back to back per class with a
+               HIP event pair around each batch on the same stream), peak =
+               8000 GB/s HBM;
+               traffic = its HBM bytes per launch from rocprofv3
+               FETCH_SIZE/WRITE_SIZE runs scaled by a plain copy's counts
+               (profiles/traffic.json; null when not measured)
   cpu_baseline the oracle's monotone stack (1 thread) on a bounded sample of
                the same inputs, timed on this host
 Timing: HIP events on the stream the kernels are launched on (torch's current
@@ -70,13 +75,17 @@ def time_solves(torch, scratch, xs, outs, steps, warmup, stream):
     t_all1.record(stream)
     torch.cuda.synchronize()
     scratch.check()
+    # per-kernel device time: each kernel of the solve launched `steps` times
+    # back to back per class, one HIP event pair per batch on the same stream
+    kt = {k: scratch.time_kernels(xs[k], outs[k], steps, stream=stream) for k in names}
+    scratch.check()
     j = 0
     for _ in range(steps):
         for k in names:
             a, b = ev[j]
             per[k].append(a.elapsed_time(b))
             j += 1
-    return per, t_all0.elapsed_time(t_all1)
+    return per, t_all0.elapsed_time(t_all1), kt
 
 
 def copy_gbs(torch, n, stream, reps=20):
@@ -153,15 +162,18 @@ def main():
     outs = {k: torch.empty(n, dtype=torch.int32, device="cuda") for k in CLASSES}
     torch.cuda.synchronize()
     with apsep_amd.Scratch(args.algo, n) as s:
-        per, total_ms = time_solves(torch, s, xs, outs, args.steps, args.warmup, stream)
+        per, total_ms, kt = time_solves(torch, s, xs, outs, args.steps, args.warmup, stream)
         algo_name = {v: k for k, v in apsep_amd.ALGOS.items()}[s.algo]
     mean = {k: sum(v) / len(v) for k, v in per.items()}
     useful = 8.0 * n
     per_class = {k: round(useful / (mean[k] * 1e-3) / 1e9, 1) for k in CLASSES}
     step_ms = total_ms / args.steps
     value = 3 * useful / (step_ms * 1e-3) / 1e9
-    kernel_ms = sum(mean.values()) / 3
-    achieved = useful / (kernel_ms * 1e-3) / 1e9
+    runs = sum(t.runs for t in kt.values())
+    tile_ms = sum(t.tile_ms for t in kt.values()) / runs
+    fix_ms = sum(t.fix_ms for t in kt.values()) / runs
+    levels_ms = sum(t.levels_ms for t in kt.values()) / runs
+    achieved = useful / (tile_ms * 1e-3) / 1e9
 
     traffic = None
     if os.path.exists(args.traffic):
@@ -169,7 +181,7 @@ def main():
             with open(args.traffic) as f:
                 t = json.load(f)
             if t.get("n") == n and t.get("algo") == algo_name:
-                traffic = t.get("bytes_per_launch")
+                traffic = t.get("tile_pass_bytes_per_launch")
         except Exception:
             traffic = None
 
@@ -195,7 +207,14 @@ def main():
         "per_class_us": {k: round(mean[k] * 1e3, 2) for k in CLASSES},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic},
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel": "k_tile_pass (mean over the 3 classes)",
+                     "algorithmic_bytes_per_launch": int(useful)},
+        "kernel_us": {"tile_pass": round(tile_ms * 1e3, 2),
+                      "level_build": round(levels_ms * 1e3, 2),
+                      "fix_pass": round(fix_ms * 1e3, 2), "launches": int(runs),
+                      "tile_pass_per_class": {k: round(t.tile_ms / t.runs * 1e3, 2)
+                                              for k, t in kt.items()}},
         "copy_gbs": round(copy_gbs(torch, n, stream), 1),
     }
     if not args.no_cpu:

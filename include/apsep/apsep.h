@@ -117,6 +117,25 @@ apsep_status apsep_scratch_query(const apsep_scratch* s, apsep_scratch_info* inf
  * No counterpart in the reference, which has no device-side error path. */
 apsep_status apsep_scratch_check(apsep_scratch* s, int32_t* device_error);
 
+/* Per-kernel device time.  No counterpart in the reference (its benchmarks
+ * time whole calls, src/main.cu:125-142); for roofline reports, which need
+ * the duration of each kernel of a solve.  Launches each kernel of a solve
+ * of d_in into d_out `reps` times back to back, in solve order (every
+ * launch of a kernel rewrites the same values, so d_out holds the answer
+ * afterwards), times each batch with HIP events on `stream` and blocks
+ * until done.  Sums are returned: divide by `runs` (= reps).  For SPT the
+ * single persistent kernel counts as the tile pass. */
+typedef struct apsep_kernel_times {
+    int64_t runs;              /* launches per kernel timed                 */
+    double tile_ms;            /* tile pass (SPT: the whole kernel)         */
+    double levels_ms;          /* level build (0 when not launched)         */
+    double fix_ms;             /* fix-up pass                               */
+} apsep_kernel_times;
+
+apsep_status apsep_time_kernels(apsep_scratch* s, const int32_t* d_in, int32_t* d_out,
+                                int64_t n, int incl, int reps, void* stream,
+                                apsep_kernel_times* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,13 @@ def test_run_and_launch_argument_checks():
     assert L.apsep_launch(1, 4096, 1 << 40, 2**31, 0) == A.ERR_TOO_LARGE
 
 
+def test_profile_argument_checks():
+    L = A.lib()
+    t = A.KernelTimes()
+    assert L.apsep_time_kernels(None, 4096, 8192, 16, 0, 1, None,
+                                ctypes.byref(t)) == A.ERR_INVALID_ARG
+
+
 def test_scratch_bytes():
     b0 = A.scratch_bytes("spt", 0)
     b1 = A.scratch_bytes("spt", 1 << 20)

@@ -204,3 +204,23 @@ def test_benchmark_configs(torch_mod, algo, n, kinds):
             # repeat on the same buffers: deterministic, nothing carried over
             got2 = solve_on(s, torch, a, False, x_dev=x, out_dev=out)
             assert np.array_equal(got, got2)
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+def test_time_kernels_leaves_valid_output(torch_mod, algo):
+    """apsep_time_kernels re-launches every kernel of the solve; the output
+    afterwards is the solve's answer and every batch has a positive time."""
+    import apsep_amd as A
+    torch = torch_mod
+    n = 3 * (1 << 20) + 17
+    a = oracle.fill("random", n, seed=11)
+    x = torch.from_numpy(a).cuda()
+    out = torch.full((n,), int(POISON), dtype=torch.int32, device="cuda")
+    with A.Scratch(algo, n) as s:
+        _log(algo, "time_kernels", n)
+        t = s.time_kernels(x, out, 3)
+        s.check()
+        assert t.runs == 3 and t.tile_ms > 0
+        if algo == "wstl":
+            assert t.fix_ms > 0
+        check(out.cpu().numpy(), a, False, "%s time_kernels" % algo)
