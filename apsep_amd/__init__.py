@@ -36,6 +36,8 @@ EXPORTS = [
     "apsep_scratch_destroy", "apsep_scratch_bytes", "apsep_run",
     "apsep_launch", "apsep_run_mode", "apsep_scratch_query",
     "apsep_scratch_check", "apsep_time_kernels",
+    "apsep_batch_plan_create", "apsep_batch_plan_destroy", "apsep_run_segmented",
+    "apsep_batch_plan_use_graph",
 ]
 
 _lib = None
@@ -85,6 +87,10 @@ def lib() -> ctypes.CDLL:
         L.apsep_scratch_check.argtypes = [vp, ctypes.POINTER(i32)]
         L.apsep_time_kernels.argtypes = [vp, vp, vp, i64, ci, ci, vp,
                                          ctypes.POINTER(KernelTimes)]
+        L.apsep_batch_plan_create.argtypes = [i64, i32, ci, ctypes.POINTER(vp)]
+        L.apsep_batch_plan_destroy.argtypes = [vp]
+        L.apsep_run_segmented.argtypes = [vp, vp, vp, vp, i32, i64, vp]
+        L.apsep_batch_plan_use_graph.argtypes = [vp, ci]
         for f in EXPORTS:
             if f != "apsep_status_string" and f != "apsep_version":
                 getattr(L, f).restype = ci
@@ -175,6 +181,63 @@ class Scratch:
     def close(self):
         if self._h is not None and self._h.value:
             lib().apsep_scratch_destroy(self._h)
+        self._h = ctypes.c_void_p(None)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchPlan:
+    """Segmented (CSR) batches of up to ``total_cap`` elements in up to
+    ``nseg_cap`` segments (apsep_batch_plan_*)."""
+
+    def __init__(self, total_cap: int, nseg_cap: int, incl: bool = False,
+                 graph: bool = False):
+        self._h = ctypes.c_void_p(None)
+        self.incl = bool(incl)
+        _check(lib().apsep_batch_plan_create(total_cap, nseg_cap, int(incl),
+                                             ctypes.byref(self._h)),
+               "apsep_batch_plan_create")
+        if graph:
+            self.use_graph(True)
+
+    def use_graph(self, enable: bool = True):
+        _check(lib().apsep_batch_plan_use_graph(self._h, int(enable)),
+               "apsep_batch_plan_use_graph")
+
+    def run_ptr(self, d_in: int, d_offsets: int, d_out: int, nseg: int, total: int,
+                stream: int = 0):
+        _check(lib().apsep_run_segmented(self._h, d_in, d_offsets, d_out, nseg, total,
+                                         stream), "apsep_run_segmented")
+
+    def run(self, x, offsets, out, total: int | None = None, stream=None):
+        """Solve every segment of ``x`` (int32) given ``offsets`` (int64, nseg + 1,
+        on the device); answers relative to each segment start go to ``out``."""
+        import torch
+        if x.dtype != torch.int32 or out.dtype != torch.int32 or offsets.dtype != torch.int64:
+            raise TypeError("apsep: int32 keys/answers and int64 offsets required")
+        if not (x.is_cuda and out.is_cuda and offsets.is_cuda):
+            raise ValueError("apsep: device tensors required")
+        if total is None:
+            total = x.numel()
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        self.run_ptr(x.data_ptr(), offsets.data_ptr(), out.data_ptr(),
+                     offsets.numel() - 1, total, stream.cuda_stream)
+        return out
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().apsep_batch_plan_destroy(self._h)
         self._h = ctypes.c_void_p(None)
 
     def __enter__(self):

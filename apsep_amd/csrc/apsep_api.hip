@@ -177,6 +177,113 @@ apsep_status apsep_time_kernels(apsep_scratch* s, const int32_t* d_in, int32_t* 
     return APSEP_OK;
 }
 
+// ---- segmented batches ----------------------------------------------------
+
+apsep_status apsep_batch_plan_create(int64_t total_cap, int32_t nseg_cap, int incl,
+                                     apsep_batch_plan** out) {
+    if (!out) return APSEP_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (total_cap < 0 || nseg_cap < 0 || (incl != 0 && incl != 1)) return APSEP_ERR_INVALID_ARG;
+    if (total_cap > INT32_MAX) return APSEP_ERR_TOO_LARGE;
+    apsep_batch_plan* p = new (std::nothrow) apsep_batch_plan();  // zeroed
+    if (!p) return APSEP_ERR_ALLOC;
+    p->total_cap = total_cap;
+    p->nseg_cap = nseg_cap;
+    apsep_status st = apsep_scratch_create(APSEP_ALGO_WSTL, total_cap, incl, &p->s);
+    if (st != APSEP_OK) {
+        delete p;
+        return st;
+    }
+    const int64_t tiles = (total_cap + TILE - 1) / TILE;
+    if (tiles > 0 && hipMalloc((void**)&p->tmap, (size_t)tiles * sizeof(int)) != hipSuccess) {
+        (void)hipGetLastError();
+        apsep_scratch_destroy(p->s);
+        delete p;
+        return APSEP_ERR_ALLOC;
+    }
+    *out = p;
+    return APSEP_OK;
+}
+
+static void drop_graph(apsep_batch_plan* p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+}
+
+apsep_status apsep_batch_plan_destroy(apsep_batch_plan* p) {
+    if (!p) return APSEP_OK;
+    drop_graph(p);
+    if (p->tmap) (void)hipFree(p->tmap);
+    apsep_scratch_destroy(p->s);
+    delete p;
+    return APSEP_OK;
+}
+
+apsep_status apsep_batch_plan_use_graph(apsep_batch_plan* p, int enable) {
+    if (!p || (enable != 0 && enable != 1)) return APSEP_ERR_INVALID_ARG;
+    if (!enable) drop_graph(p);
+    p->use_graph = enable;
+    return APSEP_OK;
+}
+
+apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
+                                 const int64_t* d_offsets, int32_t* d_out, int32_t nseg,
+                                 int64_t total, void* stream) {
+    if (!p || nseg < 0 || total < 0) return APSEP_ERR_INVALID_ARG;
+    if (total == 0) return APSEP_OK;
+    if (nseg == 0 || !d_offsets) return APSEP_ERR_INVALID_ARG;  // elements outside every segment
+    if (total > INT32_MAX) return APSEP_ERR_TOO_LARGE;
+    if (nseg > p->nseg_cap || total > p->total_cap) return APSEP_ERR_CAPACITY;
+    apsep_status st = check_io(d_in, d_out, total);
+    if (st != APSEP_OK) return st;
+    if ((uintptr_t)d_offsets & 7) return APSEP_ERR_ALIGN;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != p->s->device) {
+        (void)hipGetLastError();
+        return APSEP_ERR_DEVICE;
+    }
+    hipStream_t hs = (hipStream_t)stream;
+    if (!p->use_graph || !stream)
+        return seg_run(p, d_in, d_offsets, d_out, nseg, (int)total, hs);
+    const bool same = p->exec && p->g_in == d_in && p->g_offs == d_offsets &&
+                      p->g_out == d_out && p->g_nseg == nseg && p->g_total == total &&
+                      p->g_stream == stream;
+    if (!same) {
+        drop_graph(p);
+        if (hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            return APSEP_ERR_LAUNCH;
+        }
+        st = seg_run(p, d_in, d_offsets, d_out, nseg, (int)total, hs);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(hs, &g);
+        if (st != APSEP_OK || ec != hipSuccess || !g) {
+            (void)hipGetLastError();
+            if (g) (void)hipGraphDestroy(g);
+            return APSEP_ERR_LAUNCH;
+        }
+        p->graph = g;
+        if (hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            drop_graph(p);
+            return APSEP_ERR_LAUNCH;
+        }
+        p->g_in = d_in;
+        p->g_offs = d_offsets;
+        p->g_out = d_out;
+        p->g_nseg = nseg;
+        p->g_total = total;
+        p->g_stream = stream;
+    }
+    if (hipGraphLaunch(p->exec, hs) != hipSuccess) {
+        (void)hipGetLastError();
+        return APSEP_ERR_LAUNCH;
+    }
+    return APSEP_OK;
+}
+
 apsep_status apsep_launch(apsep_algo algo, const int32_t* d_in, int32_t* d_out,
                           int64_t n, int incl) {
     if (!valid_algo(algo) || (incl != 0 && incl != 1)) return APSEP_ERR_INVALID_ARG;

@@ -127,7 +127,40 @@ struct TileSmem {
     int wopen[NWAVE];           // open elements per wave
     int wspine[NWAVE];          // right-spine entries per wave
     int q[NWAVE][QCAP];         // per-wave queue: query (lane << 4 | slot) -> answer
+    unsigned walls[TILE / 32];  // segmented solves: segment starts inside the tile
+    int wlast[NWAVE];           // segmented solves: last segment start per wave
 };
+
+// Segmented (CSR batch) solves.  Segment k covers [offs[k], offs[k+1]) of
+// the concatenated keys; answers are relative to the segment start, -1 when
+// the segment holds no qualifying predecessor.  The solve runs on the
+// concatenation: the nearest qualifying predecessor j over the whole array
+// is the in-segment answer iff j >= the segment start (otherwise no element
+// of [start, i) qualifies, since j is the nearest one), so every answer is
+// the plain answer filtered and rebased; the fix-up bounds its prefix-min
+// and its searches by the segment start.
+struct SegIn {
+    const long long* offs;  // [nseg + 1], non-decreasing, offs[0] = 0, offs[nseg] = n
+    const int* tmap;        // [nt]: segment holding each tile's first element
+    int nseg;
+};
+
+// Per lane of a segmented tile: the start of the segment holding the lane's
+// first element, and the segment starts inside the lane (bit i = element i).
+struct SegLane {
+    int incoming;
+    unsigned wallmask;
+};
+
+// Answer `res` (global index or -1) of the lane's element i, rebased to its
+// segment (identity for plain solves).
+template <bool SEG>
+__device__ __forceinline__ int seg_rel(int res, int i, int base, const SegLane& sl) {
+    if (!SEG) return res;
+    const unsigned m = sl.wallmask & ((2u << i) - 1u);
+    const int st = m ? base + 31 - __builtin_clz(m) : sl.incoming;
+    return res >= st ? res - st : -1;
+}
 
 __device__ __forceinline__ int tile_value(const TileSmem& sm, int e) {
     const int4 w = sm.val4[(e >> 2) & (K4 - 1)][e >> 4];
@@ -382,9 +415,10 @@ __device__ __forceinline__ void load_lane(const int* __restrict__ in, int n, int
     }
 }
 
-template <bool INCL>
+template <bool INCL, bool SEG = false>
 __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, const int* v,
-                                           TileSmem& sm, const TileOut& to, const Levels& L) {
+                                           TileSmem& sm, const TileOut& to, const Levels& L,
+                                           const SegLane sl = SegLane{0, 0u}) {
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     const int wave = tid >> 6;
@@ -507,11 +541,14 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
 #pragma unroll
         for (int q = 0; q < K4; q++)
             *reinterpret_cast<int4*>(out + base + 4 * q) =
-                make_int4(res[4 * q], res[4 * q + 1], res[4 * q + 2], res[4 * q + 3]);
+                make_int4(seg_rel<SEG>(res[4 * q], 4 * q, base, sl),
+                          seg_rel<SEG>(res[4 * q + 1], 4 * q + 1, base, sl),
+                          seg_rel<SEG>(res[4 * q + 2], 4 * q + 2, base, sl),
+                          seg_rel<SEG>(res[4 * q + 3], 4 * q + 3, base, sl));
     } else {
 #pragma unroll
         for (int i = 0; i < K; i++)
-            if (i < nvalid) out[base + i] = res[i];
+            if (i < nvalid) out[base + i] = seg_rel<SEG>(res[i], i, base, sl);
     }
 
     // one scan for both counts: queries (low 16 bits), open (high 16 bits)
@@ -566,9 +603,10 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         while (m) {
             const int i = __builtin_ctz(m);
             m &= m - 1;
-            out[base + i] = (k < QCAP) ? sm.q[wave][k]
-                                       : tile_start + intile_answer<INCL>(sm, tid,
-                                                                          tile_value(sm, tid * K + i));
+            const int r = (k < QCAP) ? sm.q[wave][k]
+                                     : tile_start + intile_answer<INCL>(sm, tid,
+                                                                        tile_value(sm, tid * K + i));
+            out[base + i] = seg_rel<SEG>(r, i, base, sl);
             k++;
         }
     }
@@ -626,6 +664,56 @@ __device__ __forceinline__ void tile_step(const int* __restrict__ in, int* __res
     int v[K];
     load_lane(in, n, t, v);
     solve_tile<INCL>(out, n, t, v, sm, to, L);
+}
+
+// Segmented tile t: mark the segment starts inside the tile in LDS, give
+// every lane the start of the segment holding its first element, then solve
+// with answers rebased to their segments.
+template <bool INCL>
+__device__ __forceinline__ void tile_step_seg(const int* __restrict__ in, int* __restrict__ out,
+                                              int n, int t, TileSmem& sm, const TileOut& to,
+                                              const Levels& L, const SegIn& sg) {
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int wave = tid >> 6;
+    const int tile_start = t * TILE;
+    const int tile_end = min(tile_start + TILE, n);
+    int v[K];
+    load_lane(in, n, t, v);
+    // offsets are clamped so that malformed ones cannot send a read astray
+    const int last = max(sg.nseg - 1, 0);
+    const int k0 = min(max(sg.tmap[t], 0), last);
+    const int k1 = (t + 1 < L.nt) ? min(max(sg.tmap[t + 1], k0), last) : last;
+    const int s0 = (int)min(max(sg.offs[k0], 0ll), (long long)tile_start);
+    if (tid < TILE / 32) sm.walls[tid] = 0u;
+    __syncthreads();
+    for (int k = k0 + 1 + tid; k <= k1; k += NT) {
+        const long long w = sg.offs[k];
+        if (w > tile_start && w < tile_end) {
+            const int r = (int)w - tile_start;
+            atomicOr(&sm.walls[r >> 5], 1u << (r & 31));
+        }
+    }
+    __syncthreads();
+    const unsigned wm = (sm.walls[tid >> 1] >> ((tid & 1) * 16)) & 0xFFFFu;
+    const int base = tile_start + tid * K;
+    // last segment start at or before each lane's end: wave max-scan (DPP),
+    // then the earlier waves' maxima
+    int lw = wm ? base + 31 - __builtin_clz(wm) : s0;
+    lw = max(lw, dpp<DPP_ROW_SHR + 1>(s0, lw));
+    lw = max(lw, dpp<DPP_ROW_SHR + 2>(s0, lw));
+    lw = max(lw, dpp<DPP_ROW_SHR + 4>(s0, lw));
+    lw = max(lw, dpp<DPP_ROW_SHR + 8>(s0, lw));
+    lw = max(lw, dpp<DPP_ROW_BCAST15, 0xa>(s0, lw));
+    lw = max(lw, dpp<DPP_ROW_BCAST31, 0xc>(s0, lw));
+    const int lexcl = dpp<DPP_WAVE_SHR1>(s0, lw);
+    if (lane == WAVE - 1) sm.wlast[wave] = lw;
+    __syncthreads();
+    int incoming = lexcl;
+#pragma unroll
+    for (int w = 0; w < NWAVE - 1; w++)
+        if (w < wave) incoming = max(incoming, sm.wlast[w]);
+    solve_tile<INCL, true>(out, n, t, v, sm, to, L, SegLane{incoming, wm});
 }
 
 // ---------------------------------------------------------------------------
@@ -736,13 +824,66 @@ __device__ __forceinline__ int answer_in_tile(const int* __restrict__ in, const 
     return e >= 0 ? e : last_in_32<INCL>(in, c * CHUNK, v);
 }
 
-template <bool INCL>
+// Min over the tiles [a, b) (whole wave, uniform result): partial 64-groups
+// at both ends on each level, full groups one level up.
+__device__ __forceinline__ int range_min_tiles(const Levels& L, int a, int b) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    int m = INF;
+    int lvl = 0;
+    while (a < b) {
+        const bool up_stored = lvl + 1 < L.nlev && (lvl + 1 < 2 || L.upper);
+        if (!up_stored) {
+            for (int i = a + lane; i < b; i += WAVE) m = min(m, level_entry(L, lvl, i));
+            break;
+        }
+        const int ga = min(((a + FAN - 1) >> LOG_FAN) << LOG_FAN, b);
+        const int gb = max((b >> LOG_FAN) << LOG_FAN, ga);
+        if (a + lane < ga) m = min(m, level_entry(L, lvl, a + lane));
+        if (gb + lane < b) m = min(m, level_entry(L, lvl, gb + lane));
+        a = ga >> LOG_FAN;
+        b = gb >> LOG_FAN;
+        lvl++;
+    }
+    return wave_min(m);
+}
+
+// Min of in[s0 .. t * TILE) for s0 < t * TILE (whole wave, uniform result):
+// the rest of s0's chunk, the rest of its tile's chunk mins, then whole
+// tiles through the hierarchy.  Tile s0 / TILE precedes t, so it is complete.
+__device__ __forceinline__ int seg_prefix_min(const int* __restrict__ in, const TileOut& to,
+                                              const Levels& L, int s0, int t) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int u = s0 / TILE;
+    int m = INF;
+    int a = u;
+    if (s0 % TILE != 0) {
+        const int c0 = s0 / CHUNK;
+        if (s0 + lane < (c0 + 1) * CHUNK) m = in[s0 + lane];
+        const int c = c0 + 1 + lane;
+        if (c < (u + 1) * CPT) m = min(m, to.cmin[c]);
+        a = u + 1;
+    }
+    return min(wave_min(m), range_min_tiles(L, a, t));
+}
+
+template <bool INCL, bool SEG = false>
 __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __restrict__ out,
                                          int n, int t, const Levels& L, const TileOut& to,
-                                         int* lds) {
+                                         int* lds, const SegIn sg = SegIn{nullptr, nullptr, 0}) {
     if (t == 0) return;
     const int lane = threadIdx.x & (WAVE - 1);
     const int tile_start = t * TILE;
+    // segmented: only the open elements of the segment that runs into the
+    // tile from before (tile-relative positions < lim) can have an answer
+    int s0 = 0, lim = TILE;
+    if (SEG) {
+        const int last = max(sg.nseg - 1, 0);
+        const int k0 = min(max(sg.tmap[t], 0), last);
+        s0 = (int)min(max(sg.offs[k0], 0ll), (long long)tile_start);
+        if (s0 >= tile_start) return;
+        lim = (int)(min(max(sg.offs[k0 + 1], (long long)tile_start + 1),
+                        (long long)tile_start + TILE) - tile_start);
+    }
     const int tw = t - WAVE + lane;  // tile whose min this lane holds
     const int mw = tw >= 0 ? L.mn0[tw] : INF;
     // spines of t-1 (lanes 0..RS-1) and t-2 (lanes RS..2RS-1)
@@ -775,7 +916,7 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
             for (int i = lane; i < b1; i += WAVE) p = min(p, key_value(L.key[1][i], L.epoch));
         }
     }
-    const int P = wave_min(p);
+    const int P = SEG ? seg_prefix_min(in, to, L, s0, t) : wave_min(p);
     if (!qual<INCL>(P, v1)) return;  // every open element answers -1
     int* win = lds;
     int2* sp = reinterpret_cast<int2*>(lds + WAVE);
@@ -783,7 +924,7 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     if (lane < 2 * RS) sp[lane] = sw;
     wave_lds_sync();
     if (cnt <= QS) {
-        const bool active = lane < cnt && qual<INCL>(P, e.y);
+        const bool active = lane < cnt && qual<INCL>(P, e.y) && (!SEG || e.x < lim);
         const int v = e.y;
         // tile: the window first (tiles t-64 .. t-1) ...
         int ts = -1;
@@ -813,6 +954,7 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
             else if (ts == t - 1) r = answer_in_tile<INCL>(in, to, ts, v, sp, sc1);
             else if (ts == t - 2) r = answer_in_tile<INCL>(in, to, ts, v, sp + RS, sc2);
             else r = answer_in_tile<INCL>(in, to, ts, v, nullptr, to.scount[ts]);
+            if (SEG) r = r >= s0 ? r - s0 : -1;
             out[tile_start + e.x] = r;
         }
         return;
@@ -822,7 +964,7 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     // the tile qualifies for it (running prefix-min rm).
     int rm = INF;
     bool have = false;  // rm holds a real element (INCL: INF is a legal key)
-    const int tile_end = min(tile_start + TILE, n);
+    const int tile_end = min(tile_start + lim, n);
     for (int p0 = tile_start; p0 < tile_end; p0 += WAVE) {
         const int idx = p0 + lane;
         const bool valid = idx < tile_end;
@@ -834,7 +976,9 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
         const bool open = valid && !(ex_real && qual<INCL>(ex, xv));
         if (open && qual<INCL>(P, xv)) {
             const int ts = find_tile<INCL>(L, t, xv);
-            out[idx] = (ts < 0) ? -1 : answer_in_tile<INCL>(in, to, ts, xv, nullptr, to.scount[ts]);
+            int r = (ts < 0) ? -1 : answer_in_tile<INCL>(in, to, ts, xv, nullptr, to.scount[ts]);
+            if (SEG) r = r >= s0 ? r - s0 : -1;
+            out[idx] = r;
         }
         rm = min(rm, __shfl(inc, WAVE - 1, WAVE));
         have = true;

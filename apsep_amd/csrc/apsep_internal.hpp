@@ -29,6 +29,25 @@ struct apsep_scratch {
     apsep::GridBarrier* bar;
 };
 
+// A segmented batch: its own scratch (sized for the concatenation) plus the
+// tile -> segment map, and the captured launch sequence when graphs are on.
+struct apsep_batch_plan {
+    apsep_scratch* s;
+    int64_t total_cap;
+    int32_t nseg_cap;
+    int* tmap;             // [tiles at capacity]
+    int use_graph;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    // arguments the captured graph was recorded with
+    const void* g_in;
+    const void* g_offs;
+    void* g_out;
+    int32_t g_nseg;
+    int64_t g_total;
+    void* g_stream;
+};
+
 namespace apsep {
 
 int level_sizes(int nt, int* sizes);
@@ -37,6 +56,9 @@ apsep_status single_scratch_init(apsep_scratch* s);
 apsep_status single_run(apsep_scratch* s, const int32_t* in, int32_t* out, int n,
                         int incl, hipStream_t st);
 apsep_status single_error(apsep_scratch* s, int* code);
+apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs,
+                     int32_t* out, int nseg, int n, hipStream_t st);
+
 // One full solve, then `reps` back-to-back launches of each of its kernels
 // (each launch is idempotent), timed per kernel batch with HIP events.
 apsep_status single_time_kernels(apsep_scratch* s, const int32_t* in, int32_t* out, int n,

@@ -1,0 +1,111 @@
+// apsep_segmented.hip -- segmented (CSR batch) APSEP for gfx950: many
+// independent arrays solved by one launch sequence, with optional hipGraph
+// capture and replay of that sequence.
+//
+// The reference has no batched entry point; its callers loop over arrays
+// with one launchWSTL / launchSPT each (src/apsep.cuh:2598-2605,
+// 3069-3076).  Here the segments are concatenated and solved as one array:
+//   setup     tile -> segment map, level-1 keys reset
+//   tile pass the single-array tile solver with the segment starts inside
+//             each tile marked in LDS; answers rebased to their segments
+//   [levels]  as for one array
+//   fix-up    prefix-min and searches bounded by the segment that runs into
+//             the tile from before
+// The level-1 keys are reset by the setup kernel (a kernel, not a memset,
+// so the sequence can be captured into a graph), and the run uses a fixed
+// epoch: every replay of a captured graph then starts from clean keys.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include <new>
+
+#include "apsep_device.hpp"
+#include "apsep_internal.hpp"
+#include "../../include/apsep/apsep.h"
+
+namespace apsep {
+
+constexpr unsigned SEG_EPOCH = 1u;
+constexpr int SEG_FIX_WG = 1024;
+
+// tile -> segment holding the tile's first element; level-1 keys emptied
+__global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__ offs,
+                                                   int nseg, int* __restrict__ tmap, int nt,
+                                                   u64* __restrict__ key1, int nkey1) {
+    const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int stride = (int)(gridDim.x * blockDim.x);
+    for (int i = gid; i < nkey1; i += stride) key1[i] = ~0ull;
+    const long long cap = (long long)nt * TILE;
+    for (int k = gid; k < nseg; k += stride) {
+        const long long s = max(offs[k], 0ll);
+        const long long e = min(offs[k + 1], cap);
+        if (s >= e) continue;
+        const int t0 = (int)((s + TILE - 1) / TILE);
+        const int t1 = (int)((e - 1) / TILE);
+        for (int t = t0; t <= t1; t++) tmap[t] = k;
+    }
+}
+
+template <bool INCL>
+__global__ __launch_bounds__(NT) void k_tile_pass_seg(const int* __restrict__ in,
+                                                      int* __restrict__ out, int n,
+                                                      TileOut to, Levels L, SegIn sg) {
+    __shared__ TileSmem sm;
+    tile_step_seg<INCL>(in, out, n, (int)blockIdx.x, sm, to, L, sg);
+}
+
+__global__ __launch_bounds__(256) void k_upper_levels_seg(Levels L) { build_upper_levels(L); }
+
+template <bool INCL>
+__global__ __launch_bounds__(SEG_FIX_WG) void k_fix_pass_seg(const int* __restrict__ in,
+                                                             int* __restrict__ out, int n,
+                                                             Levels L, TileOut to, SegIn sg) {
+    __shared__ int lds[SEG_FIX_WG / WAVE][FIX_LDS_INTS];
+    const int t = (int)(blockIdx.x * (SEG_FIX_WG / WAVE) + (threadIdx.x >> 6));
+    if (t >= L.nt) return;
+    fix_tile<INCL, true>(in, out, n, t, L, to, lds[threadIdx.x >> 6], sg);
+}
+
+template <bool INCL>
+static apsep_status seg_launch(apsep_batch_plan* p, const int* in, const long long* offs,
+                               int* out, int nseg, int n, hipStream_t st) {
+    apsep_scratch* s = p->s;
+    const int nt = (n + TILE - 1) / TILE;
+    Levels L;
+    int sz[MAX_LEVELS];
+    L.nt = nt;
+    L.nlev = level_sizes(nt, sz);
+    L.mn0 = s->mn0;
+    L.upper = (L.nlev > 2 && sz[1] > UPPER_MIN) ? 1 : 0;
+    for (int l = 0; l < MAX_LEVELS; l++) {
+        L.n[l] = l < L.nlev ? sz[l] : 0;
+        L.key[l] = (l >= 1 && l < L.nlev) ? s->key[l] : nullptr;
+    }
+    L.epoch = SEG_EPOCH;
+    TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
+    SegIn sg{offs, p->tmap, nseg};
+    const int nkey1 = L.nlev > 1 ? sz[1] : 0;
+    const int work = nseg > nkey1 ? nseg : nkey1;
+    int blocks = (work + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_seg_setup, dim3(blocks), dim3(256), 0, st, offs, nseg, p->tmap, nt,
+                       nkey1 ? s->key[1] : nullptr, nkey1);
+    hipLaunchKernelGGL(k_tile_pass_seg<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L, sg);
+    if (nt > 1) {
+        if (L.upper) hipLaunchKernelGGL(k_upper_levels_seg, dim3(1), dim3(256), 0, st, L);
+        constexpr int tpw = SEG_FIX_WG / WAVE;
+        hipLaunchKernelGGL(k_fix_pass_seg<INCL>, dim3((nt + tpw - 1) / tpw), dim3(SEG_FIX_WG), 0,
+                           st, in, out, n, L, to, sg);
+    }
+    return hipGetLastError() == hipSuccess ? APSEP_OK : APSEP_ERR_LAUNCH;
+}
+
+apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs,
+                     int32_t* out, int nseg, int n, hipStream_t st) {
+    const long long* o = reinterpret_cast<const long long*>(offs);
+    return p->s->incl ? seg_launch<true>(p, in, o, out, nseg, n, st)
+                      : seg_launch<false>(p, in, o, out, nseg, n, st);
+}
+
+}  // namespace apsep

@@ -136,6 +136,36 @@ apsep_status apsep_time_kernels(apsep_scratch* s, const int32_t* d_in, int32_t* 
                                 int64_t n, int incl, int reps, void* stream,
                                 apsep_kernel_times* out);
 
+/* ---- Segmented (CSR) batches ---------------------------------------------
+ * No counterpart in the reference, whose callers solve many arrays with one
+ * launchWSTL / launchSPT each (src/apsep.cuh:2598-2605, 3069-3076), paying
+ * an allocation, a launch sequence and a device synchronisation per array.
+ * A plan holds the device scratch for batches of up to total_cap elements
+ * in up to nseg_cap segments, with the predicate fixed at creation. */
+typedef struct apsep_batch_plan apsep_batch_plan;
+
+apsep_status apsep_batch_plan_create(int64_t total_cap, int32_t nseg_cap, int incl,
+                                     apsep_batch_plan** out);
+apsep_status apsep_batch_plan_destroy(apsep_batch_plan* p);
+
+/* Segment k covers d_in[d_offsets[k] .. d_offsets[k+1]) (nseg + 1 int64
+ * device offsets, non-decreasing, d_offsets[0] = 0, d_offsets[nseg] =
+ * total; total is passed so that no device read is needed to size the
+ * launch).  d_out[i] = index of the nearest qualifying predecessor of i
+ * inside i's segment, relative to the segment start, or -1.  Async on
+ * `stream`.  Malformed offsets give unspecified answers but no access
+ * outside d_in, d_out and d_offsets[0 .. nseg]. */
+apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
+                                 const int64_t* d_offsets, int32_t* d_out, int32_t nseg,
+                                 int64_t total, void* stream);
+
+/* With graphs on and a non-null stream, the first apsep_run_segmented
+ * records its launch sequence into a hipGraph and later calls with the same
+ * pointers, nseg, total and stream replay it with one hipGraphLaunch (the
+ * offsets and keys may change between calls; other arguments re-record).
+ * The sequence holds no memset node. */
+apsep_status apsep_batch_plan_use_graph(apsep_batch_plan* p, int enable);
+
 #ifdef __cplusplus
 }
 #endif

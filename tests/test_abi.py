@@ -98,6 +98,20 @@ def test_profile_argument_checks():
                                 ctypes.byref(t)) == A.ERR_INVALID_ARG
 
 
+def test_batch_plan_argument_checks():
+    L = A.lib()
+    h = ctypes.c_void_p()
+    assert L.apsep_batch_plan_create(100, 4, 0, None) == A.ERR_INVALID_ARG
+    assert L.apsep_batch_plan_create(-1, 4, 0, ctypes.byref(h)) == A.ERR_INVALID_ARG
+    assert L.apsep_batch_plan_create(100, -4, 0, ctypes.byref(h)) == A.ERR_INVALID_ARG
+    assert L.apsep_batch_plan_create(100, 4, 2, ctypes.byref(h)) == A.ERR_INVALID_ARG
+    assert L.apsep_batch_plan_create(2**31, 4, 0, ctypes.byref(h)) == A.ERR_TOO_LARGE
+    assert not h.value
+    assert L.apsep_batch_plan_destroy(None) == A.OK
+    assert L.apsep_batch_plan_use_graph(None, 1) == A.ERR_INVALID_ARG
+    assert L.apsep_run_segmented(None, 4096, 8192, 16384, 1, 10, None) == A.ERR_INVALID_ARG
+
+
 def test_scratch_bytes():
     b0 = A.scratch_bytes("spt", 0)
     b1 = A.scratch_bytes("spt", 1 << 20)

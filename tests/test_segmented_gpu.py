@@ -1,0 +1,147 @@
+"""Segmented (CSR batch) solves through the C ABI, bit-exact against the
+oracle's per-segment monotone stack (answers relative to each segment
+start).  Covers empty and one-element segments, segments that start inside
+a lane, at lane/chunk/tile edges and span many tiles, every input class,
+both predicates, and hipGraph capture/replay with new data between replays.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+POISON = np.int32(np.uint32(0xABABABAB).view(np.int32))
+
+
+def _log(*a):
+    print("[segmented]", *a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def _offsets(lengths):
+    off = np.zeros(len(lengths) + 1, dtype=np.int64)
+    np.cumsum(np.asarray(lengths, dtype=np.int64), out=off[1:])
+    return off
+
+
+def _keys(kind, lengths, seed):
+    """Concatenated keys, one class per segment (or `kind` for all)."""
+    parts = []
+    rng = np.random.default_rng(seed)
+    kinds = ("random", "descending", "ascending", "dup16", "dup2")
+    for j, L in enumerate(lengths):
+        k = kind if kind != "mixed" else kinds[rng.integers(len(kinds))]
+        parts.append(oracle.fill(k, int(L), seed=seed + j) if L else np.zeros(0, np.int32))
+    return np.concatenate(parts).astype(np.int32) if parts else np.zeros(0, np.int32)
+
+
+def _solve(plan, torch, a, off, stream=None):
+    x = torch.from_numpy(a).cuda()
+    o = torch.from_numpy(off).cuda()
+    out = torch.full((max(a.shape[0], 1),), int(POISON), dtype=torch.int32, device="cuda")
+    plan.run(x, o, out, total=a.shape[0], stream=stream)
+    torch.cuda.synchronize()
+    return out[:a.shape[0]].cpu().numpy()
+
+
+def _check(got, a, off, incl, what):
+    exp = oracle.apsep_segmented(a, off, incl)
+    if not np.array_equal(got, exp):
+        bad = np.nonzero(got != exp)[0]
+        i = int(bad[0])
+        k = int(np.searchsorted(off, i, side="right") - 1)
+        raise AssertionError("%s: %d mismatches, first i=%d (segment %d at %d) got=%d exp=%d"
+                             % (what, bad.size, i, k, off[k], got[i], exp[i]))
+
+
+EDGE_LENGTHS = [0, 1, 2, 15, 16, 17, 0, 63, 64, 65, 4095, 4096, 4097, 1, 0, 8191, 12289,
+                3, 70000, 5, 0, 4096 * 70 + 3, 7, 1]
+
+
+@pytest.mark.parametrize("incl", [False, True])
+@pytest.mark.parametrize("kind", ["random", "descending", "ascending", "dup2", "mixed"])
+def test_edge_lengths(torch_mod, kind, incl):
+    import apsep_amd as A
+    lengths = EDGE_LENGTHS
+    a = _keys(kind, lengths, seed=3)
+    off = _offsets(lengths)
+    with A.BatchPlan(a.shape[0], len(lengths), incl) as p:
+        _log(kind, incl, a.shape[0])
+        _check(_solve(p, torch_mod, a, off), a, off, incl, "edge %s incl=%d" % (kind, incl))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_many_small_segments(torch_mod, seed):
+    """Thousands of segments shorter than a lane / a tile (many starts per
+    tile, several per lane)."""
+    import apsep_amd as A
+    rng = np.random.default_rng(seed)
+    lengths = rng.integers(0, 40, size=5000)
+    a = _keys("mixed", lengths, seed)
+    off = _offsets(lengths)
+    with A.BatchPlan(a.shape[0], len(lengths)) as p:
+        _log("small", seed, len(lengths), a.shape[0])
+        _check(_solve(p, torch_mod, a, off), a, off, False, "small seed=%d" % seed)
+
+
+def test_loguniform_batch(torch_mod):
+    """A scaled-down batched configuration: log-uniform lengths in
+    [64, 2^18], class uniform over random/descending/ascending."""
+    import apsep_amd as A
+    off = oracle.loguniform_offsets(2024, 600, 64.0, float(1 << 18))
+    lengths = np.diff(off)
+    a = _keys("mixed", lengths, 9)
+    with A.BatchPlan(a.shape[0], len(lengths)) as p:
+        _log("loguniform", len(lengths), a.shape[0])
+        _check(_solve(p, torch_mod, a, off), a, off, False, "loguniform")
+        # the same plan again, smaller batch (capacity is an upper bound)
+        off2 = off[:301]
+        a2 = a[:int(off2[-1])]
+        _check(_solve(p, torch_mod, a2, off2), a2, off2, False, "loguniform prefix")
+
+
+def test_one_segment_equals_plain_solve(torch_mod):
+    import apsep_amd as A
+    n = (1 << 22) + 77
+    a = oracle.fill("random", n, seed=5)
+    off = np.array([0, n], dtype=np.int64)
+    with A.BatchPlan(n, 1) as p:
+        got = _solve(p, torch_mod, a, off)
+    assert np.array_equal(got, oracle.apsep(a))
+
+
+def test_graph_replay_with_new_data(torch_mod):
+    """Capture once, replay with new keys and new offsets in the same
+    buffers, eager work and synchronisation between replays; every replay
+    checked."""
+    import apsep_amd as A
+    torch = torch_mod
+    total, nseg = 1 << 21, 900
+    x = torch.empty(total, dtype=torch.int32, device="cuda")
+    o = torch.empty(nseg + 1, dtype=torch.int64, device="cuda")
+    out = torch.empty(total, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.Stream()
+    with A.BatchPlan(total, nseg, graph=True) as p:
+        for rep in range(5):
+            rng = np.random.default_rng(100 + rep)
+            cuts = np.sort(rng.integers(0, total, size=nseg - 1))
+            off = np.concatenate([[0], cuts, [total]]).astype(np.int64)
+            a = _keys("mixed", np.diff(off), 50 + rep)
+            x.copy_(torch.from_numpy(a))
+            o.copy_(torch.from_numpy(off))
+            out.fill_(int(POISON))
+            torch.cuda.synchronize()
+            _log("graph replay", rep)
+            with torch.cuda.stream(stream):
+                p.run(x, o, out, total=total, stream=stream)
+            stream.synchronize()
+            _check(out.cpu().numpy(), a, off, False, "graph replay %d" % rep)
