@@ -135,6 +135,130 @@ def cpu_baseline(torch, xs, outs, budget_s: float):
     }, ok
 
 
+def batch_inputs(torch, nseg: int, lo: int, hi: int, seed: int):
+    """Log-uniform segment lengths floor(exp(U(ln lo, ln hi))), class per
+    segment uniform over random / descending / ascending; keys built on the
+    device."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    lengths = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), size=nseg))).astype(np.int64)
+    cls = rng.integers(0, 3, size=nseg)
+    off = np.zeros(nseg + 1, dtype=np.int64)
+    np.cumsum(lengths, out=off[1:])
+    total = int(off[-1])
+    dev = "cuda"
+    L = torch.from_numpy(lengths).to(dev)
+    seg = torch.repeat_interleave(torch.arange(nseg, device=dev), L)
+    local = torch.arange(total, device=dev, dtype=torch.int64) - torch.from_numpy(off[:-1]).to(dev)[seg]
+    c = torch.from_numpy(cls).to(dev)[seg]
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    rnd = torch.randint(0, 2**31 - 1, (total,), generator=g, dtype=torch.int32, device=dev)
+    desc = (L[seg] - local).to(torch.int32)
+    x = torch.where(c == 0, rnd, torch.where(c == 1, desc, local.to(torch.int32)))
+    del seg, local, c, rnd, desc
+    return x.contiguous(), off, lengths, cls
+
+
+def bench_batch(args, torch, apsep_amd):
+    """Segmented batch: 8192 segments with log-uniform lengths in [64, 2^20]
+    (about 8.9e8 keys), one class per segment.  Reports the batched solve
+    (eager and hipGraph replay) against one solve per array (the reference's
+    calling pattern: a persistent scratch and apsep_run per array, and
+    apsep_launch -- allocate, solve, synchronise, free -- per array)."""
+    import numpy as np
+    stream = torch.cuda.current_stream()
+    x, off, lengths, cls = batch_inputs(torch, args.nseg, 64, 1 << 20, 2024)
+    total, nseg = int(off[-1]), args.nseg
+    offs = torch.from_numpy(off).cuda()
+    out = torch.empty(total, dtype=torch.int32, device="cuda")
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps):
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(reps):
+            fn()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / reps
+
+    res = {}
+    with apsep_amd.BatchPlan(total, nseg) as p:
+        run = lambda: p.run(x, offs, out, total=total, stream=stream)
+        for _ in range(args.warmup):
+            run()
+        res["eager_ms"] = timed(run, args.steps)
+        p.use_graph(True)
+        run()
+        res["graph_ms"] = timed(run, args.steps)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+
+    # one solve per array, each array in its own 16-byte aligned slot
+    pstart = np.zeros(nseg + 1, dtype=np.int64)
+    np.cumsum((lengths + 3) // 4 * 4, out=pstart[1:])
+    ptotal = int(pstart[-1])
+    seg = torch.repeat_interleave(torch.arange(nseg, device="cuda"),
+                                  torch.from_numpy(lengths).cuda())
+    dst = torch.arange(total, device="cuda") - offs[:-1][seg] + torch.from_numpy(pstart[:-1]).cuda()[seg]
+    xp = torch.zeros(ptotal, dtype=torch.int32, device="cuda")
+    xp[dst] = x
+    outp = torch.empty(ptotal, dtype=torch.int32, device="cuda")
+    del seg, dst
+    xb, ob = xp.data_ptr(), outp.data_ptr()
+    with apsep_amd.Scratch("auto", int(lengths.max())) as s:
+        sh = stream.cuda_stream
+
+        def per_array():
+            for k in range(nseg):
+                s.run_ptr(xb + 4 * int(pstart[k]), ob + 4 * int(pstart[k]), int(lengths[k]), sh)
+        per_array()
+        res["per_array_run_ms"] = timed(per_array, 1)
+
+    def per_launch():
+        for k in range(nseg):
+            apsep_amd.launch("auto", xb + 4 * int(pstart[k]), ob + 4 * int(pstart[k]),
+                             int(lengths[k]))
+    res["per_array_launch_ms"] = timed(per_launch, 1)
+
+    useful = 8.0 * total
+    best = min(res["eager_ms"], res["graph_ms"])
+    rec = {
+        "metric": "batched useful GB/s (8192 segments, log-uniform lengths 64..2^20)",
+        "value": round(useful / (best * 1e-3) / 1e9, 1),
+        "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(best, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (numpy seed 2024 lengths/classes, device-generated keys)",
+        "config": {"workload": "segmented CSR batch, %d segments, %d keys" % (nseg, total),
+                   "segments": nseg, "total": total},
+        "eager_ms": round(res["eager_ms"], 3), "graph_ms": round(res["graph_ms"], 3),
+        "per_array_run_ms": round(res["per_array_run_ms"], 2),
+        "per_array_launch_ms": round(res["per_array_launch_ms"], 2),
+        "speedup_vs_per_array_run": round(res["per_array_run_ms"] / best, 2),
+        "speedup_vs_per_array_launch": round(res["per_array_launch_ms"] / best, 2),
+    }
+    if not args.no_cpu:
+        import oracle
+        host = x.cpu().numpy()
+        threads = len(os.sched_getaffinity(0))
+        t0 = time.perf_counter()
+        exp = oracle.apsep_segmented(host, off, False, threads=threads)
+        secs = time.perf_counter() - t0
+        rec["cpu_baseline"] = {
+            "value": round(useful / secs / 1e9, 3), "unit": "GB/s", "cores": threads,
+            "kind": "port",
+            "sample": "the whole batch once, segments as jobs on a %d-thread pool "
+                      "(oracle/apsep_oracle.c -O3), %.2f s" % (threads, secs)}
+        rec["parity_checked"] = bool(np.array_equal(exp, got))
+        outp_h = outp.cpu().numpy()
+        rec["per_array_parity_checked"] = bool(all(
+            np.array_equal(outp_h[pstart[k]:pstart[k] + lengths[k]], got[off[k]:off[k + 1]])
+            for k in range(0, nseg, 97)))
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +270,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="rocprofv3 FETCH/WRITE summary (bytes per launch)")
+    ap.add_argument("--workload", default="single", choices=["single", "batch"],
+                    help="single: the plain N=2^25 solve (headline); batch: the "
+                         "segmented CSR batch")
+    ap.add_argument("--nseg", type=int, default=8192)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not "
@@ -155,6 +283,10 @@ def main():
 
     import torch
     import apsep_amd
+
+    if args.workload == "batch":
+        bench_batch(args, torch, apsep_amd)
+        return
 
     n = args.n
     stream = torch.cuda.current_stream()
