@@ -38,6 +38,10 @@ EXPORTS = [
     "apsep_scratch_check", "apsep_time_kernels",
     "apsep_batch_plan_create", "apsep_batch_plan_destroy", "apsep_run_segmented",
     "apsep_batch_plan_use_graph",
+    "apsep_server_create", "apsep_server_destroy", "apsep_server_submit",
+    "apsep_server_poll", "apsep_server_job_info", "apsep_server_release",
+    "apsep_server_drain", "apsep_server_submit_many", "apsep_server_done_upto",
+    "apsep_server_job_times", "apsep_server_release_upto",
 ]
 
 _lib = None
@@ -56,6 +60,16 @@ class KernelTimes(ctypes.Structure):
     """apsep_kernel_times (include/apsep/apsep.h): summed device time per kernel."""
     _fields_ = [("runs", ctypes.c_int64), ("tile_ms", ctypes.c_double),
                 ("levels_ms", ctypes.c_double), ("fix_ms", ctypes.c_double)]
+
+
+class JobInfo(ctypes.Structure):
+    """apsep_job_info (include/apsep/apsep.h)."""
+    _fields_ = [("state", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("n", ctypes.c_int64), ("submit_us", ctypes.c_double),
+                ("done_us", ctypes.c_double), ("d_answers", ctypes.c_void_p)]
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
 
 
 class ApsepError(RuntimeError):
@@ -91,6 +105,18 @@ def lib() -> ctypes.CDLL:
         L.apsep_batch_plan_destroy.argtypes = [vp]
         L.apsep_run_segmented.argtypes = [vp, vp, vp, vp, i32, i64, vp]
         L.apsep_batch_plan_use_graph.argtypes = [vp, ci]
+        L.apsep_server_create.argtypes = [i64, i64, i32, ci, ctypes.POINTER(vp)]
+        L.apsep_server_destroy.argtypes = [vp]
+        L.apsep_server_submit.argtypes = [vp, vp, i64, ctypes.POINTER(i64)]
+        L.apsep_server_poll.argtypes = [vp, ctypes.POINTER(i32)]
+        L.apsep_server_job_info.argtypes = [vp, i64, ctypes.POINTER(JobInfo)]
+        L.apsep_server_release.argtypes = [vp, i64]
+        L.apsep_server_drain.argtypes = [vp]
+        L.apsep_server_submit_many.argtypes = [vp, vp, vp, i32, ctypes.POINTER(i64),
+                                               ctypes.POINTER(i32)]
+        L.apsep_server_done_upto.argtypes = [vp, ctypes.POINTER(i64)]
+        L.apsep_server_job_times.argtypes = [vp, i64, i32, vp, vp]
+        L.apsep_server_release_upto.argtypes = [vp, i64]
         for f in EXPORTS:
             if f != "apsep_status_string" and f != "apsep_version":
                 getattr(L, f).restype = ci
@@ -238,6 +264,119 @@ class BatchPlan:
     def close(self):
         if self._h is not None and self._h.value:
             lib().apsep_batch_plan_destroy(self._h)
+        self._h = ctypes.c_void_p(None)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_hiplib = None
+
+
+def _hip():
+    """The HIP runtime (for device-to-device copies out of a server's pool)."""
+    global _hiplib
+    if _hiplib is None:
+        h = ctypes.CDLL("libamdhip64.so")
+        h.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                     ctypes.c_int, ctypes.c_void_p]
+        h.hipMemcpyAsync.restype = ctypes.c_int
+        _hiplib = h
+    return _hiplib
+
+
+class Server:
+    """Serving engine (apsep_server_*): a ``pool_bytes`` device pool, budget
+    admission and continuous batching into segmented solves."""
+
+    def __init__(self, pool_bytes: int, max_batch: int, max_batch_jobs: int = 4096,
+                 incl: bool = False):
+        self._h = ctypes.c_void_p(None)
+        _check(lib().apsep_server_create(pool_bytes, max_batch, max_batch_jobs, int(incl),
+                                         ctypes.byref(self._h)), "apsep_server_create")
+
+    def submit(self, d_keys: int, n: int):
+        """Job id, or None when the pool has no room (admission refused)."""
+        jid = ctypes.c_int64(-1)
+        st = lib().apsep_server_submit(self._h, d_keys, n, ctypes.byref(jid))
+        if st == ERR_BUDGET:
+            return None
+        _check(st, "apsep_server_submit")
+        return jid.value
+
+    def poll(self) -> int:
+        k = ctypes.c_int32(0)
+        _check(lib().apsep_server_poll(self._h, ctypes.byref(k)), "apsep_server_poll")
+        return k.value
+
+    def info(self, job_id: int) -> JobInfo:
+        ji = JobInfo()
+        _check(lib().apsep_server_job_info(self._h, job_id, ctypes.byref(ji)),
+               "apsep_server_job_info")
+        return ji
+
+    def answers(self, job_id: int, out=None):
+        """Copy a done job's answers into ``out`` (int32 device tensor of n;
+        allocated when None) on torch's current stream."""
+        import torch
+        info = self.info(job_id)
+        if info.state != JOB_DONE:
+            raise ApsepError(ERR_INVALID_ARG, "job %d is not done" % job_id)
+        if out is None:
+            out = torch.empty(info.n, dtype=torch.int32, device="cuda")
+        if info.n:
+            st = torch.cuda.current_stream().cuda_stream
+            if _hip().hipMemcpyAsync(out.data_ptr(), info.d_answers, info.n * 4,
+                                     3, st) != 0:  # hipMemcpyDeviceToDevice
+                raise ApsepError(ERR_LAUNCH, "hipMemcpyAsync")
+        return out
+
+    def release(self, job_id: int):
+        _check(lib().apsep_server_release(self._h, job_id), "apsep_server_release")
+
+    def drain(self):
+        _check(lib().apsep_server_drain(self._h), "apsep_server_drain")
+
+    def submit_many(self, ptrs, lengths):
+        """Submit jobs in order (numpy uint64 pointers / int64 lengths);
+        returns (first id, number accepted)."""
+        import numpy as np
+        p = np.ascontiguousarray(ptrs, dtype=np.uint64)
+        n = np.ascontiguousarray(lengths, dtype=np.int64)
+        first, acc = ctypes.c_int64(0), ctypes.c_int32(0)
+        _check(lib().apsep_server_submit_many(self._h, p.ctypes.data, n.ctypes.data, p.size,
+                                              ctypes.byref(first), ctypes.byref(acc)),
+               "apsep_server_submit_many")
+        return first.value, acc.value
+
+    def done_upto(self) -> int:
+        d = ctypes.c_int64(0)
+        _check(lib().apsep_server_done_upto(self._h, ctypes.byref(d)), "apsep_server_done_upto")
+        return d.value
+
+    def job_times(self, first_id: int, count: int):
+        import numpy as np
+        sub = np.zeros(count, dtype=np.float64)
+        done = np.zeros(count, dtype=np.float64)
+        _check(lib().apsep_server_job_times(self._h, first_id, count, sub.ctypes.data,
+                                            done.ctypes.data), "apsep_server_job_times")
+        return sub, done
+
+    def release_upto(self, job_id: int):
+        _check(lib().apsep_server_release_upto(self._h, job_id), "apsep_server_release_upto")
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().apsep_server_destroy(self._h)
         self._h = ctypes.c_void_p(None)
 
     def __enter__(self):

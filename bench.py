@@ -259,6 +259,120 @@ def bench_batch(args, torch, apsep_amd):
     print(json.dumps(rec), flush=True)
 
 
+def bench_serve(args, torch, apsep_amd):
+    """Served stream of independent jobs: Poisson arrivals at swept rates,
+    lengths log-uniform in [2^10, 2^24], classes 60/20/20 random /
+    descending / ascending, a 64 GB device pool.  Jobs' keys are resident in
+    device memory when they arrive (a fixed set of distinct payloads, reused
+    round-robin); latency runs from the scheduled arrival (so a job refused
+    by admission waits in the client backlog) to the moment the server
+    observes its batch finished."""
+    import numpy as np
+    rng = np.random.default_rng(7)
+    P = args.payloads
+    lengths = np.floor(np.exp(rng.uniform(np.log(1 << 10), np.log(1 << 24), size=P))).astype(np.int64)
+    cls = rng.choice(3, size=P, p=[0.6, 0.2, 0.2])
+    off = np.zeros(P + 1, dtype=np.int64)
+    np.cumsum(lengths, out=off[1:])
+    total = int(off[-1])
+    dev = "cuda"
+    L = torch.from_numpy(lengths).to(dev)
+    seg = torch.repeat_interleave(torch.arange(P, device=dev), L)
+    local = torch.arange(total, device=dev, dtype=torch.int64) - torch.from_numpy(off[:-1]).to(dev)[seg]
+    c = torch.from_numpy(cls).to(dev)[seg]
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    rnd = torch.randint(0, 2**31 - 1, (total,), generator=g, dtype=torch.int32, device=dev)
+    src = torch.where(c == 0, rnd, torch.where(c == 1, (L[seg] - local).to(torch.int32),
+                                               local.to(torch.int32))).contiguous()
+    del seg, local, c, rnd
+    torch.cuda.synchronize()
+    base = src.data_ptr()
+    ptr = [base + 4 * int(off[k]) for k in range(P)]
+    lens = [int(x) for x in lengths]
+    rates = [float(r) for r in args.rates.split(",")]
+    J = args.jobs
+    sweep = []
+    sample_answers = {}
+    ptr_arr = np.array(ptr, dtype=np.uint64)
+    len_arr = np.array(lens, dtype=np.int64)
+    with apsep_amd.Server(64 << 30, 1 << 28, 4096) as srv:
+        for rate in rates:
+            arr = (time.monotonic_ns() / 1e3 + 2e5
+                   + np.cumsum(rng.exponential(1e6 / rate, size=J)))
+            which = np.arange(J) % P
+            jp, jl = ptr_arr[which], len_arr[which]
+            nxt = rel = 0      # jobs submitted / released
+            id0 = None         # id of this rate's first job (ids are consecutive)
+            lat = np.zeros(J)
+            refused = 0
+            while rel < J:
+                now = time.monotonic_ns() / 1e3
+                m = int(np.searchsorted(arr, now, side="right"))
+                if m > nxt:
+                    first, acc = srv.submit_many(jp[nxt:m], jl[nxt:m])
+                    if id0 is None:
+                        id0 = first
+                    refused += acc < m - nxt
+                    nxt += acc
+                srv.poll()
+                if id0 is not None:
+                    d = srv.done_upto() - id0
+                    if d > rel:
+                        _, done = srv.job_times(id0 + rel, d - rel)
+                        lat[rel:d] = done - arr[rel:d]
+                        if rate == rates[-1]:
+                            for j in range(rel, d):
+                                if j % 997 == 0:
+                                    sample_answers[j % P] = srv.answers(id0 + j).cpu().numpy()
+                        t_last = float(done[-1])
+                        srv.release_upto(id0 + d)
+                        rel = d
+                if nxt < J and rel == nxt:
+                    wait = (arr[nxt] - time.monotonic_ns() / 1e3) * 1e-6
+                    if wait > 2e-4:
+                        time.sleep(min(wait - 1e-4, 1e-3))
+            span_s = (t_last - arr[0]) * 1e-6
+            done_elems = int(jl.sum())
+            lat_ms = lat * 1e-3
+            sweep.append({"offered_jobs_per_s": rate,
+                          "jobs_per_s": round(J / span_s, 1),
+                          "keys_per_s": round(done_elems / span_s, 1),
+                          "useful_GBps": round(8.0 * done_elems / span_s / 1e9, 1),
+                          "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+                          "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+                          "admission_refusals": int(refused)})
+            print("serve rate %.0f/s: %s" % (rate, json.dumps(sweep[-1])), file=sys.stderr,
+                  flush=True)
+    best = max(sweep, key=lambda r: r["jobs_per_s"])
+    rec = {
+        "metric": "served jobs/s (Poisson arrivals, lengths log-uniform 2^10..2^24, 64 GB pool)",
+        "value": best["jobs_per_s"], "unit": "jobs/s", "n_gpus": 1, "steps": J,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (numpy seed 7 lengths/classes/arrivals, device-generated keys, "
+                "%d distinct payloads reused round-robin)" % P,
+        "config": {"workload": "served stream, %d jobs per rate, mean %.0f keys"
+                               % (J, float(lengths.mean())), "rates": rates},
+        "sweep": sweep,
+    }
+    if not args.no_cpu:
+        import oracle
+        host = src.cpu().numpy()
+        threads = len(os.sched_getaffinity(0))
+        t0 = time.perf_counter()
+        exp = oracle.apsep_segmented(host, off, False, threads=threads)
+        secs = time.perf_counter() - t0
+        rec["cpu_baseline"] = {
+            "value": round(P / secs, 1), "unit": "jobs/s", "cores": threads, "kind": "port",
+            "sample": "the %d distinct payloads once, as independent jobs on a %d-thread "
+                      "pool (oracle/apsep_oracle.c -O3), %.2f s" % (P, threads, secs)}
+        rec["parity_checked"] = bool(all(
+            np.array_equal(v, exp[off[k]:off[k + 1]]) for k, v in sample_answers.items()))
+        rec["parity_jobs_checked"] = len(sample_answers)
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,10 +384,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="rocprofv3 FETCH/WRITE summary (bytes per launch)")
-    ap.add_argument("--workload", default="single", choices=["single", "batch"],
+    ap.add_argument("--workload", default="single", choices=["single", "batch", "serve"],
                     help="single: the plain N=2^25 solve (headline); batch: the "
-                         "segmented CSR batch")
+                         "segmented CSR batch; serve: the serving engine")
     ap.add_argument("--nseg", type=int, default=8192)
+    ap.add_argument("--jobs", type=int, default=20000, help="serve: jobs per rate")
+    ap.add_argument("--payloads", type=int, default=1000, help="serve: distinct jobs")
+    ap.add_argument("--rates", default="10000,50000,150000,400000",
+                    help="serve: offered jobs/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not "
@@ -286,6 +404,9 @@ def main():
 
     if args.workload == "batch":
         bench_batch(args, torch, apsep_amd)
+        return
+    if args.workload == "serve":
+        bench_serve(args, torch, apsep_amd)
         return
 
     n = args.n

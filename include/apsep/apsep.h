@@ -166,6 +166,55 @@ apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
  * The sequence holds no memset node. */
 apsep_status apsep_batch_plan_use_graph(apsep_batch_plan* p, int enable);
 
+/* ---- Serving: HBM pool, budget admission, continuous batching ----------
+ * No counterpart in the reference (its launch* allocate and synchronise per
+ * call, src/apsep.cuh:2598-2605, 3069-3076); for callers that solve a
+ * stream of independent arrays.  A server owns a pool of pool_bytes of
+ * device memory (half keys, half answers), its own stream and a batch plan;
+ * one thread drives it.
+ *   submit   admits a job of n keys if its slot fits in the free part of
+ *            the pool (APSEP_ERR_BUDGET otherwise: nothing is queued); the
+ *            device keys must stay valid until the job is done
+ *   poll     non-blocking: retires finished batches, then packs queued jobs
+ *            (up to max_batch keys / max_batch_jobs jobs, at most two
+ *            batches in flight) into segmented solves
+ *   job_info state (0 queued, 1 running, 2 done), submit/done times in
+ *            microseconds of a monotonic clock, and the device answers
+ *            (valid until release)
+ *   release  frees a done job's slot; drain blocks until every job is done */
+typedef struct apsep_server apsep_server;
+
+typedef struct apsep_job_info {
+    int32_t state;
+    int32_t reserved;
+    int64_t n;
+    double submit_us;
+    double done_us;
+    const int32_t* d_answers;
+} apsep_job_info;
+
+apsep_status apsep_server_create(int64_t pool_bytes, int64_t max_batch,
+                                 int32_t max_batch_jobs, int incl, apsep_server** out);
+apsep_status apsep_server_destroy(apsep_server* s);
+apsep_status apsep_server_submit(apsep_server* s, const int32_t* d_keys, int64_t n,
+                                 int64_t* job_id);
+apsep_status apsep_server_poll(apsep_server* s, int32_t* batches_launched);
+apsep_status apsep_server_job_info(apsep_server* s, int64_t job_id, apsep_job_info* info);
+apsep_status apsep_server_release(apsep_server* s, int64_t job_id);
+apsep_status apsep_server_drain(apsep_server* s);
+
+/* Bulk forms for high job rates: submit a list (the accepted prefix gets
+ * consecutive ids from *first_id; admission stops at the first job that
+ * does not fit), the id below which every job is done, submit/done times
+ * of a range of ids, and release of every job below an id (all done). */
+apsep_status apsep_server_submit_many(apsep_server* s, const int32_t* const* d_keys,
+                                      const int64_t* n, int32_t count, int64_t* first_id,
+                                      int32_t* accepted);
+apsep_status apsep_server_done_upto(apsep_server* s, int64_t* done_upto);
+apsep_status apsep_server_job_times(apsep_server* s, int64_t first_id, int32_t count,
+                                    double* submit_us, double* done_us);
+apsep_status apsep_server_release_upto(apsep_server* s, int64_t id);
+
 #ifdef __cplusplus
 }
 #endif

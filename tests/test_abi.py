@@ -112,6 +112,24 @@ def test_batch_plan_argument_checks():
     assert L.apsep_run_segmented(None, 4096, 8192, 16384, 1, 10, None) == A.ERR_INVALID_ARG
 
 
+def test_server_argument_checks():
+    L = A.lib()
+    h = ctypes.c_void_p()
+    jid = ctypes.c_int64()
+    assert L.apsep_server_create(1 << 20, 1000, 16, 0, None) == A.ERR_INVALID_ARG
+    assert L.apsep_server_create(8, 1000, 16, 0, ctypes.byref(h)) == A.ERR_INVALID_ARG
+    assert L.apsep_server_create(1 << 20, 0, 16, 0, ctypes.byref(h)) == A.ERR_INVALID_ARG
+    assert L.apsep_server_create(1 << 20, 1000, 0, 0, ctypes.byref(h)) == A.ERR_INVALID_ARG
+    assert L.apsep_server_create(1 << 20, 1000, 16, 5, ctypes.byref(h)) == A.ERR_INVALID_ARG
+    assert L.apsep_server_create(1 << 40, 2**31, 16, 0, ctypes.byref(h)) == A.ERR_TOO_LARGE
+    assert not h.value
+    assert L.apsep_server_destroy(None) == A.OK
+    assert L.apsep_server_submit(None, 4096, 10, ctypes.byref(jid)) == A.ERR_INVALID_ARG
+    assert L.apsep_server_poll(None, None) == A.ERR_INVALID_ARG
+    assert L.apsep_server_release(None, 0) == A.ERR_INVALID_ARG
+    assert L.apsep_server_drain(None) == A.ERR_INVALID_ARG
+
+
 def test_scratch_bytes():
     b0 = A.scratch_bytes("spt", 0)
     b1 = A.scratch_bytes("spt", 1 << 20)

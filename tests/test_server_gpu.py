@@ -1,0 +1,138 @@
+"""Serving engine through the C ABI: budget admission, ring reuse with
+wrap-around, continuous batching, and every job's answers bit-exact against
+the oracle's monotone stack."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(*a):
+    print("[server]", *a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def _jobs(torch, lengths, seed):
+    kinds = ("random", "descending", "ascending", "dup16")
+    rng = np.random.default_rng(seed)
+    host, dev = [], []
+    for j, n in enumerate(lengths):
+        a = oracle.fill(kinds[rng.integers(len(kinds))], int(n), seed=seed * 1000 + j) \
+            if n else np.zeros(0, np.int32)
+        host.append(a)
+        dev.append(torch.from_numpy(a).cuda() if n else None)
+    return host, dev
+
+
+def _answers(s, jid):
+    return s.answers(jid).cpu().numpy()
+
+
+def test_batch_of_jobs_all_correct(torch_mod):
+    import apsep_amd as A
+    torch = torch_mod
+    rng = np.random.default_rng(1)
+    lengths = list(rng.integers(0, 200000, size=60)) + [0, 1, 2, 3, 4, 5, 4096, 4097]
+    host, dev = _jobs(torch, lengths, 1)
+    torch.cuda.synchronize()
+    with A.Server(1 << 30, 1 << 24, 64) as s:
+        ids = []
+        for k, n in enumerate(lengths):
+            jid = s.submit(dev[k].data_ptr() if n else 0, int(n))
+            assert jid is not None
+            ids.append(jid)
+            if k % 7 == 0:
+                s.poll()
+        s.drain()
+        for k, jid in enumerate(ids):
+            info = s.info(jid)
+            assert info.state == A.JOB_DONE and info.done_us >= info.submit_us
+            got = _answers(s, jid)
+            assert np.array_equal(got, oracle.apsep(host[k])), "job %d n=%d" % (k, lengths[k])
+            s.release(jid)
+
+
+def test_budget_admission_and_wraparound(torch_mod):
+    """A pool that holds a few jobs at a time: submissions are refused until
+    finished jobs are released; slots wrap around the ring."""
+    import apsep_amd as A
+    torch = torch_mod
+    rng = np.random.default_rng(2)
+    lengths = list(rng.integers(1000, 150000, size=80))
+    host, dev = _jobs(torch, lengths, 2)
+    torch.cuda.synchronize()
+    pool = 8 * 400000  # 400k keys + 400k answers
+    with A.Server(pool, 400000, 16) as s:
+        pending = list(range(len(lengths)))
+        live = []
+        refused = 0
+        checked = 0
+        while pending or live:
+            while pending:
+                k = pending[0]
+                jid = s.submit(dev[k].data_ptr(), int(lengths[k]))
+                if jid is None:
+                    refused += 1
+                    break
+                live.append((k, jid))
+                pending.pop(0)
+            s.poll()
+            # release finished jobs in order
+            while live:
+                k, jid = live[0]
+                info = s.info(jid)
+                if info.state != A.JOB_DONE:
+                    if not pending:
+                        s.drain()
+                    break
+                got = _answers(s, jid)
+                assert np.array_equal(got, oracle.apsep(host[k])), "job %d" % k
+                checked += 1
+                s.release(jid)
+                live.pop(0)
+        _log("refused", refused, "checked", checked)
+        assert checked == len(lengths) and refused > 0
+
+
+def test_oversized_job_rejected(torch_mod):
+    import apsep_amd as A
+    torch = torch_mod
+    x = torch.zeros(5000, dtype=torch.int32, device="cuda")
+    with A.Server(1 << 20, 4096, 8) as s:
+        with pytest.raises(A.ApsepError) as e:
+            s.submit(x.data_ptr(), 5000)
+        assert e.value.status == A.ERR_CAPACITY
+
+
+def test_bulk_submit_and_release(torch_mod):
+    import apsep_amd as A
+    torch = torch_mod
+    rng = np.random.default_rng(3)
+    lengths = list(rng.integers(0, 90000, size=50))
+    host, dev = _jobs(torch, lengths, 3)
+    torch.cuda.synchronize()
+    ptrs = np.array([d.data_ptr() if d is not None else 0 for d in dev], dtype=np.uint64)
+    with A.Server(1 << 28, 1 << 22, 32) as s:
+        first, acc = s.submit_many(ptrs, lengths)
+        assert acc == len(lengths)
+        s.drain()
+        assert s.done_upto() == first + acc
+        sub, done = s.job_times(first, acc)
+        assert np.all(done >= sub) and np.all(sub > 0)
+        for k in range(acc):
+            assert np.array_equal(_answers(s, first + k), oracle.apsep(host[k])), "job %d" % k
+        s.release_upto(first + acc)
+        # the pool is empty again: a job as large as the batch limit fits
+        big = torch.zeros(1 << 22, dtype=torch.int32, device="cuda")
+        assert s.submit(big.data_ptr(), 1 << 22) is not None
+        s.drain()
