@@ -42,7 +42,6 @@ static apsep_algo resolve_algo(apsep_algo a) {
     // the multi-launch solver is the fastest on gfx950: the persistent one
     // pays for agent-scope fences at every grid barrier
     if (a == APSEP_ALGO_AUTO) return APSEP_ALGO_WSTL;
-    if (a == APSEP_ALGO_LOOKBACK) return APSEP_ALGO_SPT;  // not built yet
     return a;
 }
 
@@ -51,7 +50,7 @@ apsep_status apsep_scratch_bytes(apsep_algo algo, int64_t n_cap, int incl,
     if (!bytes || !valid_algo(algo) || n_cap < 0 || (incl != 0 && incl != 1))
         return APSEP_ERR_INVALID_ARG;
     if (n_cap > INT32_MAX) return APSEP_ERR_TOO_LARGE;
-    *bytes = n_cap > 0 ? single_scratch_bytes(n_cap) : 0;
+    *bytes = n_cap > 0 ? single_scratch_bytes(n_cap, resolve_algo(algo)) : 0;
     return APSEP_OK;
 }
 

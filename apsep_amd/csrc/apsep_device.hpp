@@ -781,6 +781,18 @@ __device__ int find_tile_wave(const Levels& L, int tb, int v) {
 // round, an older tile one more (its spine), and only a spine longer than
 // RS falls back to chunk mins + elements.
 // ---------------------------------------------------------------------------
+// Control words of a look-back run (device memory): run tag, tile ticket
+// counter, error word; each on its own 128-byte line.
+struct LbCtl {
+    unsigned epoch;
+    unsigned pad0[31];
+    unsigned ticket;
+    unsigned pad1[31];
+    int err;
+    unsigned pad2[31];
+};
+constexpr int LB_LEVELS = MAX_LEVELS;
+
 constexpr int FIX_LDS_INTS = WAVE + 4 * RS;  // tile-min window + 2 spines
 
 template <bool INCL>

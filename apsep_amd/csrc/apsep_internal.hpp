@@ -27,6 +27,12 @@ struct apsep_scratch {
     int2* spine;
     int coop_grid;        // SPT persistent grid size (occupancy x CUs)
     apsep::GridBarrier* bar;
+    // look-back runs: control words and the published (tagged) words
+    apsep::LbCtl* lb_ctl;
+    unsigned long long* lb_M;
+    unsigned long long* lb_S;
+    unsigned long long* lb_CM;
+    unsigned long long* lb_G[apsep::LB_LEVELS];
 };
 
 // A segmented batch: its own scratch (sized for the concatenation) plus the
@@ -51,7 +57,9 @@ struct apsep_batch_plan {
 namespace apsep {
 
 int level_sizes(int nt, int* sizes);
-int64_t single_scratch_bytes(int64_t n_cap);
+int64_t single_scratch_bytes(int64_t n_cap, apsep_algo algo);
+apsep_status run_lookback(apsep_scratch* s, const int32_t* in, int32_t* out, int n, int incl,
+                          hipStream_t st);
 apsep_status single_scratch_init(apsep_scratch* s);
 apsep_status single_run(apsep_scratch* s, const int32_t* in, int32_t* out, int n,
                         int incl, hipStream_t st);

@@ -94,7 +94,7 @@ int level_sizes(int nt, int* sizes) {
     return nl;
 }
 
-int64_t single_scratch_bytes(int64_t n_cap) {
+int64_t single_scratch_bytes(int64_t n_cap, apsep_algo algo) {
     const int nt = (int)((n_cap + TILE - 1) / TILE);
     int sz[MAX_LEVELS];
     const int nl = level_sizes(nt < 1 ? 1 : nt, sz);
@@ -106,6 +106,13 @@ int64_t single_scratch_bytes(int64_t n_cap) {
     b += (round4(nt) + 4) * 4;                        // scount
     b += ((int64_t)nt * RS + 2) * 8;                  // spine
     b += (int64_t)sizeof(GridBarrier);                // grid barrier
+    if (algo == APSEP_ALGO_LOOKBACK) {
+        b += (int64_t)sizeof(LbCtl);
+        b += 2 * (round4(nt) + 4) * 8;                // tile mins, inclusive prefixes
+        b += (round4((int64_t)nt * CPT) + 4) * 8;      // chunk mins
+        for (int l = 1; l < nl; l++) b += (round4(sz[l]) + 4) * 8;  // group mins
+        b += 256 * (4 + LB_LEVELS);
+    }
     return b + 256 * 12;                              // alignment slack
 }
 
@@ -121,7 +128,7 @@ apsep_status single_scratch_init(apsep_scratch* s) {
     if (nt == 0) return APSEP_OK;
     int sz[MAX_LEVELS];
     const int nl = level_sizes(nt, sz);
-    s->bytes = single_scratch_bytes(s->n_cap);
+    s->bytes = single_scratch_bytes(s->n_cap, s->algo);
     if (hipMalloc(&s->base, (size_t)s->bytes) != hipSuccess) {
         s->base = nullptr;
         (void)hipGetLastError();
@@ -137,6 +144,13 @@ apsep_status single_scratch_init(apsep_scratch* s) {
     s->scount = (int*)carve(p, (round4(nt) + 4) * 4);
     s->spine = (int2*)carve(p, ((int64_t)nt * RS + 2) * 8);
     s->bar = (GridBarrier*)carve(p, (int64_t)sizeof(GridBarrier));
+    if (s->algo == APSEP_ALGO_LOOKBACK) {
+        s->lb_ctl = (LbCtl*)carve(p, (int64_t)sizeof(LbCtl));
+        s->lb_M = (u64*)carve(p, (round4(nt) + 4) * 8);
+        s->lb_S = (u64*)carve(p, (round4(nt) + 4) * 8);
+        s->lb_CM = (u64*)carve(p, (round4((int64_t)nt * CPT) + 4) * 8);
+        for (int l = 1; l < nl; l++) s->lb_G[l] = (u64*)carve(p, (round4(sz[l]) + 4) * 8);
+    }
     if ((char*)p > (char*)s->base + s->bytes) return APSEP_ERR_ALLOC;  // sizing bug guard
     if (hipMemset(s->base, 0, (size_t)s->bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -307,6 +321,8 @@ apsep_status single_run(apsep_scratch* s, const int32_t* in, int32_t* out, int n
         return incl ? run_wstl<true>(s, in, out, n, st) : run_wstl<false>(s, in, out, n, st);
     case APSEP_ALGO_SPT:
         return incl ? run_spt<true>(s, in, out, n, st) : run_spt<false>(s, in, out, n, st);
+    case APSEP_ALGO_LOOKBACK:
+        return run_lookback(s, in, out, n, incl, st);
     default:
         return APSEP_ERR_INVALID_ARG;
     }
@@ -314,6 +330,17 @@ apsep_status single_run(apsep_scratch* s, const int32_t* in, int32_t* out, int n
 
 apsep_status single_error(apsep_scratch* s, int* code) {
     *code = 0;
+    if (s->lb_ctl) {
+        int lerr = 0;
+        if (hipMemcpy(&lerr, &s->lb_ctl->err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            (void)hipGetLastError();
+            return APSEP_ERR_LAUNCH;
+        }
+        if (lerr) {
+            *code = 2;  // a look-back wait gave up; the next run resets the word
+            return APSEP_OK;
+        }
+    }
     if (!s->bar) return APSEP_OK;
     int err = 0;
     if (hipMemcpy(&err, &s->bar->err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {

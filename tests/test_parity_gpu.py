@@ -22,7 +22,7 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-ALGOS = ["wstl", "spt"]
+ALGOS = ["wstl", "spt", "lookback"]
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 POISON = np.int32(np.uint32(0xABABABAB).view(np.int32))
 
@@ -224,3 +224,35 @@ def test_time_kernels_leaves_valid_output(torch_mod, algo):
         if algo == "wstl":
             assert t.fix_ms > 0
         check(out.cpu().numpy(), a, False, "%s time_kernels" % algo)
+
+
+def test_lookback_graph_replay(torch_mod):
+    """The look-back sequence keeps no host-baked state: capture it once
+    with torch.cuda.graph, then replay with new keys in the same buffer,
+    eager work and synchronisation between replays; every replay checked."""
+    import apsep_amd as A
+    torch = torch_mod
+    n = (1 << 21) + 5
+    x = torch.empty(n, dtype=torch.int32, device="cuda")
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    with A.Scratch("lookback", n) as s:
+        x.copy_(torch.from_numpy(oracle.fill("random", n, seed=1)))
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            s.run(x, out)  # warm-up outside the capture
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            s.run(x, out)
+        for rep, kind in enumerate(["random", "descending", "dup16", "ascending", "random"]):
+            a = oracle.fill(kind, n, seed=10 + rep)
+            x.copy_(torch.from_numpy(a))
+            out.fill_(int(POISON))
+            torch.cuda.synchronize()
+            _log("lookback graph replay", rep, kind)
+            g.replay()
+            torch.cuda.synchronize()
+            s.check()
+            check(out.cpu().numpy(), a, False, "lookback replay %d %s" % (rep, kind))
