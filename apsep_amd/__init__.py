@@ -37,7 +37,7 @@ EXPORTS = [
     "apsep_launch", "apsep_run_mode", "apsep_scratch_query",
     "apsep_scratch_check", "apsep_time_kernels",
     "apsep_batch_plan_create", "apsep_batch_plan_destroy", "apsep_run_segmented",
-    "apsep_batch_plan_use_graph",
+    "apsep_batch_plan_use_graph", "apsep_batch_plan_tune",
     "apsep_server_create", "apsep_server_destroy", "apsep_server_submit",
     "apsep_server_poll", "apsep_server_job_info", "apsep_server_release",
     "apsep_server_drain", "apsep_server_submit_many", "apsep_server_done_upto",
@@ -105,6 +105,9 @@ def lib() -> ctypes.CDLL:
         L.apsep_batch_plan_destroy.argtypes = [vp]
         L.apsep_run_segmented.argtypes = [vp, vp, vp, vp, i32, i64, vp]
         L.apsep_batch_plan_use_graph.argtypes = [vp, ci]
+        L.apsep_batch_plan_tune.argtypes = [vp, vp, vp, vp, i32, i64, vp, i32,
+                                            ctypes.POINTER(ctypes.c_float),
+                                            ctypes.POINTER(ctypes.c_float)]
         L.apsep_server_create.argtypes = [i64, i64, i32, ci, ctypes.POINTER(vp)]
         L.apsep_server_destroy.argtypes = [vp]
         L.apsep_server_submit.argtypes = [vp, vp, i64, ctypes.POINTER(i64)]
@@ -239,6 +242,21 @@ class BatchPlan:
     def use_graph(self, enable: bool = True):
         _check(lib().apsep_batch_plan_use_graph(self._h, int(enable)),
                "apsep_batch_plan_use_graph")
+
+    def tune(self, x, offsets, out, total: int | None = None, stream=None, reps: int = 3):
+        """Time direct launches against graph replay on this batch and keep
+        the faster; returns (eager_ms, graph_ms)."""
+        import torch
+        if total is None:
+            total = x.numel()
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        e, g = ctypes.c_float(0), ctypes.c_float(0)
+        _check(lib().apsep_batch_plan_tune(self._h, x.data_ptr(), offsets.data_ptr(),
+                                           out.data_ptr(), offsets.numel() - 1, total,
+                                           stream.cuda_stream, reps, ctypes.byref(e),
+                                           ctypes.byref(g)), "apsep_batch_plan_tune")
+        return e.value, g.value
 
     def run_ptr(self, d_in: int, d_offsets: int, d_out: int, nseg: int, total: int,
                 stream: int = 0):

@@ -283,6 +283,48 @@ apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
     return APSEP_OK;
 }
 
+apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
+                                   const int64_t* d_offsets, int32_t* d_out, int32_t nseg,
+                                   int64_t total, void* stream, int32_t reps,
+                                   float* eager_ms, float* graph_ms) {
+    if (!p || reps < 1) return APSEP_ERR_INVALID_ARG;
+    if (!stream) return APSEP_ERR_INVALID_ARG;  // graphs need a capturable stream
+    hipStream_t hs = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) {
+        (void)hipGetLastError();
+        return APSEP_ERR_DEVICE;
+    }
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipEventDestroy(e0);
+        return APSEP_ERR_DEVICE;
+    }
+    float t[2] = {0.f, 0.f};
+    apsep_status st = APSEP_OK;
+    for (int g = 0; g < 2 && st == APSEP_OK; g++) {
+        st = apsep_batch_plan_use_graph(p, g);
+        // one untimed call (records the graph), then the timed ones
+        if (st == APSEP_OK) st = apsep_run_segmented(p, d_in, d_offsets, d_out, nseg, total, stream);
+        if (st == APSEP_OK && hipEventRecord(e0, hs) != hipSuccess) st = APSEP_ERR_LAUNCH;
+        for (int r = 0; r < reps && st == APSEP_OK; r++)
+            st = apsep_run_segmented(p, d_in, d_offsets, d_out, nseg, total, stream);
+        if (st == APSEP_OK &&
+            (hipEventRecord(e1, hs) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+             hipEventElapsedTime(&t[g], e0, e1) != hipSuccess))
+            st = APSEP_ERR_LAUNCH;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != APSEP_OK) {
+        (void)hipGetLastError();
+        return st;
+    }
+    if (eager_ms) *eager_ms = t[0] / reps;
+    if (graph_ms) *graph_ms = t[1] / reps;
+    return apsep_batch_plan_use_graph(p, t[1] < t[0] ? 1 : 0);
+}
+
 apsep_status apsep_launch(apsep_algo algo, const int32_t* d_in, int32_t* d_out,
                           int64_t n, int incl) {
     if (!valid_algo(algo) || (incl != 0 && incl != 1)) return APSEP_ERR_INVALID_ARG;

@@ -184,14 +184,19 @@ def bench_batch(args, torch, apsep_amd):
         return ev0.elapsed_time(ev1) / reps
 
     res = {}
+    stream = torch.cuda.Stream()  # graphs need a capturable (non-null) stream
     with apsep_amd.BatchPlan(total, nseg) as p:
         run = lambda: p.run(x, offs, out, total=total, stream=stream)
-        for _ in range(args.warmup):
+        with torch.cuda.stream(stream):
+            tuned = p.tune(x, offs, out, total=total, stream=stream)
+            res["tuned_mode"] = "graph" if tuned[1] < tuned[0] else "eager"
+            p.use_graph(False)
+            for _ in range(args.warmup):
+                run()
+            res["eager_ms"] = timed(run, args.steps)
+            p.use_graph(True)
             run()
-        res["eager_ms"] = timed(run, args.steps)
-        p.use_graph(True)
-        run()
-        res["graph_ms"] = timed(run, args.steps)
+            res["graph_ms"] = timed(run, args.steps)
     torch.cuda.synchronize()
     got = out.cpu().numpy()
 
@@ -234,6 +239,7 @@ def bench_batch(args, torch, apsep_amd):
         "config": {"workload": "segmented CSR batch, %d segments, %d keys" % (nseg, total),
                    "segments": nseg, "total": total},
         "eager_ms": round(res["eager_ms"], 3), "graph_ms": round(res["graph_ms"], 3),
+        "autotuned_mode": res["tuned_mode"],
         "per_array_run_ms": round(res["per_array_run_ms"], 2),
         "per_array_launch_ms": round(res["per_array_launch_ms"], 2),
         "speedup_vs_per_array_run": round(res["per_array_run_ms"] / best, 2),
@@ -373,6 +379,55 @@ def bench_serve(args, torch, apsep_amd):
     print(json.dumps(rec), flush=True)
 
 
+def bench_sweep(args, torch, apsep_amd):
+    """Useful GB/s of the plain solve for N = 2^14 .. 2^27 on the three
+    classes, cache-warm (repeated solves of the same array) and with the
+    256 MB last-level cache flushed before every solve (a 1 GiB buffer is
+    rewritten between timed solves; only the solve is inside the events)."""
+    stream = torch.cuda.current_stream()
+    flush = torch.empty(1 << 28, dtype=torch.int32, device="cuda")
+    rows = []
+    with apsep_amd.Scratch(args.algo, 1 << 27) as s:
+        for lg in range(14, 28):
+            n = 1 << lg
+            xs = make_inputs(torch, n)
+            out = torch.empty(n, dtype=torch.int32, device="cuda")
+            row = {"n": n}
+            for k in CLASSES:
+                for _ in range(3):
+                    s.run(xs[k], out, stream=stream)
+                evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                       for _ in range(2 * args.steps)]
+                torch.cuda.synchronize()
+                for i in range(args.steps):  # warm
+                    evs[i][0].record(stream)
+                    s.run(xs[k], out, stream=stream)
+                    evs[i][1].record(stream)
+                for i in range(args.steps, 2 * args.steps):  # flushed
+                    flush.fill_(i)
+                    evs[i][0].record(stream)
+                    s.run(xs[k], out, stream=stream)
+                    evs[i][1].record(stream)
+                torch.cuda.synchronize()
+                warm = sorted(a.elapsed_time(b) for a, b in evs[:args.steps])[args.steps // 2]
+                cold = sorted(a.elapsed_time(b) for a, b in evs[args.steps:])[args.steps // 2]
+                row[k] = {"warm_GBps": round(8.0 * n / (warm * 1e-3) / 1e9, 1),
+                          "flushed_GBps": round(8.0 * n / (cold * 1e-3) / 1e9, 1),
+                          "warm_us": round(warm * 1e3, 2), "flushed_us": round(cold * 1e3, 2)}
+            s.check()
+            rows.append(row)
+            print("sweep n=2^%d %s" % (lg, json.dumps(row)), file=sys.stderr, flush=True)
+    best = max(r["random"]["warm_GBps"] for r in rows)
+    print(json.dumps({
+        "metric": "useful GB/s sweep, N=2^14..2^27 x 3 classes (median of %d solves)" % args.steps,
+        "value": rows[-3]["random"]["warm_GBps"], "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 3, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (device-generated, as the headline)",
+        "config": {"workload": "plain solve size sweep", "algo": args.algo},
+        "best_random_warm_GBps": best, "rows": rows}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,9 +439,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="rocprofv3 FETCH/WRITE summary (bytes per launch)")
-    ap.add_argument("--workload", default="single", choices=["single", "batch", "serve"],
+    ap.add_argument("--workload", default="single", choices=["single", "batch", "serve", "sweep"],
                     help="single: the plain N=2^25 solve (headline); batch: the "
-                         "segmented CSR batch; serve: the serving engine")
+                         "segmented CSR batch; serve: the serving engine; sweep: "
+                         "sizes 2^14..2^27, cache-warm and flushed")
     ap.add_argument("--nseg", type=int, default=8192)
     ap.add_argument("--jobs", type=int, default=20000, help="serve: jobs per rate")
     ap.add_argument("--payloads", type=int, default=1000, help="serve: distinct jobs")
@@ -407,6 +463,9 @@ def main():
         return
     if args.workload == "serve":
         bench_serve(args, torch, apsep_amd)
+        return
+    if args.workload == "sweep":
+        bench_sweep(args, torch, apsep_amd)
         return
 
     n = args.n

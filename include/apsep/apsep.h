@@ -166,6 +166,15 @@ apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
  * The sequence holds no memset node. */
 apsep_status apsep_batch_plan_use_graph(apsep_batch_plan* p, int enable);
 
+/* Autotune a plan for a batch shape: times `reps` direct-launch runs and
+ * `reps` graph replays of this batch on `stream` (non-null), keeps the
+ * faster mode and reports both mean times (blocking; d_out holds the
+ * answer afterwards). */
+apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
+                                   const int64_t* d_offsets, int32_t* d_out, int32_t nseg,
+                                   int64_t total, void* stream, int32_t reps,
+                                   float* eager_ms, float* graph_ms);
+
 /* ---- Serving: HBM pool, budget admission, continuous batching ----------
  * No counterpart in the reference (its launch* allocate and synchronise per
  * call, src/apsep.cuh:2598-2605, 3069-3076); for callers that solve a
