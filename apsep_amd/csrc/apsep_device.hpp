@@ -117,19 +117,25 @@ struct TileOut {
 // component i & 3: lane-contiguous dwordx4 rows, so every ds_*_b128 of the
 // tile pass is bank-conflict free.  Lane-min windows live inside 16-lane
 // rows (the reach of a DPP row shift): win[s][l] = min of the lane mins of
-// lanes max(l - 2^s + 1, row start) .. l for s = 0..3, win[4][l] = min of
-// row start .. l.
+// lanes max(l - 2^s + 1, row start) .. l for s = 0..3.  Kept under 22.5 KB
+// so that seven tiles fit a CU.
 struct TileSmem {
     int4 val4[K4][NT];          // the tile's values
-    int win[5][NT];             // lane-min windows inside rows
+    int win[4][NT];             // lane-min windows inside rows
     int4 rmin[NWAVE];           // row mins of each wave
     int wmin[NWAVE];            // wave mins
     int wopen[NWAVE];           // open elements per wave
     int wspine[NWAVE];          // right-spine entries per wave
-    int q[NWAVE][QCAP];         // per-wave queue: query (lane << 4 | slot) -> answer
-    unsigned walls[TILE / 32];  // segmented solves: segment starts inside the tile
     int wlast[NWAVE];           // segmented solves: last segment start per wave
+    union {
+        // per-wave queue: query (lane << 4 | slot) -> tile-relative answer
+        unsigned short q[NWAVE][QCAP];
+        // segmented solves, before the tile is solved: segment starts
+        // inside the tile
+        unsigned walls[TILE / 32];
+    };
 };
+static_assert(sizeof(TileSmem) <= 22528, "seven tiles per CU");
 
 // Segmented (CSR batch) solves.  Segment k covers [offs[k], offs[k+1]) of
 // the concatenated keys; answers are relative to the segment start, -1 when
@@ -368,7 +374,11 @@ template <bool INCL>
 __device__ __forceinline__ int intile_answer(const TileSmem& sm, int qt, int val) {
     const int wq = qt >> 6, ql = qt & (WAVE - 1), r = ql >> 4;
     int p;
-    if ((ql & 15) > 0 && qual<INCL>(sm.win[4][qt - 1], val)) {
+    // min of the own row's lanes before qt: two 8-lane windows
+    const int rl = ql & 15;
+    const int rowpre = rl == 0 ? INF
+                     : min(sm.win[3][qt - 1], rl > 8 ? sm.win[3][qt - 9] : INF);
+    if (rl > 0 && qual<INCL>(rowpre, val)) {
         p = qt - 1;
     } else {
         int4 rm = sm.rmin[wq];
@@ -455,7 +465,6 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     sm.win[1][tid] = w2;
     sm.win[2][tid] = w4;
     sm.win[3][tid] = w8;
-    sm.win[4][tid] = w16;
     // window of LPC lanes = one 64-element chunk
     if ((lane & (LPC - 1)) == LPC - 1) to.cmin[(size_t)t * CPT + (tid >> LOG_LPC)] = w4;
     int winc = min(w16, dpp<DPP_ROW_BCAST15, 0xa>(INF, w16));
@@ -589,13 +598,13 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         while (m) {
             const int i = __builtin_ctz(m);
             m &= m - 1;
-            if (k < QCAP) sm.q[wave][k] = (tid << 4) | i;
+            if (k < QCAP) sm.q[wave][k] = (unsigned short)((tid << 4) | i);
             k++;
         }
         wave_lds_sync();
         for (int r = lane; r < min(qtotal, QCAP); r += WAVE) {
             const int item = sm.q[wave][r];
-            sm.q[wave][r] = tile_start + intile_answer<INCL>(sm, item >> 4, tile_value(sm, item));
+            sm.q[wave][r] = (unsigned short)intile_answer<INCL>(sm, item >> 4, tile_value(sm, item));
         }
         wave_lds_sync();
         m = qmask;
@@ -603,9 +612,9 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         while (m) {
             const int i = __builtin_ctz(m);
             m &= m - 1;
-            const int r = (k < QCAP) ? sm.q[wave][k]
-                                     : tile_start + intile_answer<INCL>(sm, tid,
-                                                                        tile_value(sm, tid * K + i));
+            const int r = tile_start + ((k < QCAP) ? (int)sm.q[wave][k]
+                                                   : intile_answer<INCL>(sm, tid,
+                                                                         tile_value(sm, tid * K + i)));
             out[base + i] = seg_rel<SEG>(r, i, base, sl);
             k++;
         }
