@@ -5,8 +5,8 @@
 // (strict) or in[j] <= in[i] (INCL), else -1.
 //
 // Work decomposition (wave64 throughout):
-//   tile   = one workgroup of NT = 256 lanes (4 waves) x K = 16 consecutive
-//            elements per lane = 4096 elements, loaded with four dwordx4
+//   tile   = one workgroup of NT = 512 lanes (8 waves) x K = 16 consecutive
+//            elements per lane = 8192 elements, loaded with four dwordx4
 //            loads per lane.
 //   chunk  = 64 consecutive elements (4 lanes); its min is published for the
 //            inter-tile search.
@@ -50,15 +50,15 @@
 namespace apsep {
 
 constexpr int WAVE = 64;
-constexpr int NT = 256;            // lanes per tile workgroup
+constexpr int NT = 512;            // lanes per tile workgroup
 constexpr int K = 16;              // consecutive elements per lane
 constexpr int K4 = K / 4;          // dwordx4 per lane
-constexpr int TILE = NT * K;       // 4096 elements per tile
-constexpr int NWAVE = NT / WAVE;   // 4 waves per tile
+constexpr int TILE = NT * K;       // 8192 elements per tile
+constexpr int NWAVE = NT / WAVE;   // 8 waves per tile
 constexpr int CHUNK = 64;          // elements per published chunk min
 constexpr int LPC = CHUNK / K;     // lanes per chunk (4)
 constexpr int LOG_LPC = 2;
-constexpr int CPT = TILE / CHUNK;  // 64 chunk mins per tile
+constexpr int CPT = TILE / CHUNK;  // 128 chunk mins per tile
 constexpr int QS = 32;             // recorded open elements per tile
 constexpr int RS = 32;             // recorded right-spine entries per tile
 constexpr int QCAP = 192;          // per-wave LDS queue of cross-lane queries
@@ -70,7 +70,7 @@ constexpr int INF = INT_MAX;
 #define SPT_WAVES_PER_EU 4         // occupancy the persistent kernel is pinned at
 
 static_assert(K == 16 && LPC == 4 && (1 << LOG_LPC) == LPC, "tile shape");
-static_assert(CPT % 32 == 0, "chunk-min search works in blocks of 32");
+static_assert(CPT % 16 == 0, "chunk-min search works in blocks of 16");
 
 typedef unsigned long long u64;
 
@@ -117,8 +117,8 @@ struct TileOut {
 // component i & 3: lane-contiguous dwordx4 rows, so every ds_*_b128 of the
 // tile pass is bank-conflict free.  Lane-min windows live inside 16-lane
 // rows (the reach of a DPP row shift): win[s][l] = min of the lane mins of
-// lanes max(l - 2^s + 1, row start) .. l for s = 0..3.  Kept under 22.5 KB
-// so that seven tiles fit a CU.
+// lanes max(l - 2^s + 1, row start) .. l for s = 0..3 (44 KB: three tiles
+// per CU).
 struct TileSmem {
     int4 val4[K4][NT];          // the tile's values
     int win[4][NT];             // lane-min windows inside rows
@@ -135,7 +135,7 @@ struct TileSmem {
         unsigned walls[TILE / 32];
     };
 };
-static_assert(sizeof(TileSmem) <= 22528, "seven tiles per CU");
+
 
 // Segmented (CSR batch) solves.  Segment k covers [offs[k], offs[k+1]) of
 // the concatenated keys; answers are relative to the segment start, -1 when
@@ -838,8 +838,24 @@ __device__ __forceinline__ int answer_in_tile(const int* __restrict__ in, const 
     }
     if (pos >= 0) return s * TILE + pos;
     // deeper than the recorded spine: chunk mins, then the chunk
-    int c = last_in_32<INCL>(to.cmin, s * CPT + 32, v);
-    if (c < 0) c = last_in_32<INCL>(to.cmin, s * CPT, v);
+    int c = -1;
+    if (CPT % 32 == 0) {
+        for (int b = s * CPT + CPT - 32; b >= s * CPT && c < 0; b -= 32)
+            c = last_in_32<INCL>(to.cmin, b, v);
+    } else {
+        for (int b = s * CPT + CPT - 16; b >= s * CPT && c < 0; b -= 16) {
+            int4 w[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) w[k] = *reinterpret_cast<const int4*>(to.cmin + b + 4 * k);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                c = qual<INCL>(w[k].x, v) ? b + 4 * k : c;
+                c = qual<INCL>(w[k].y, v) ? b + 4 * k + 1 : c;
+                c = qual<INCL>(w[k].z, v) ? b + 4 * k + 2 : c;
+                c = qual<INCL>(w[k].w, v) ? b + 4 * k + 3 : c;
+            }
+        }
+    }
     if (c < 0) return -1;  // unreachable: the tile's min qualifies
     const int e = last_in_32<INCL>(in, c * CHUNK + 32, v);
     return e >= 0 ? e : last_in_32<INCL>(in, c * CHUNK, v);
@@ -880,8 +896,7 @@ __device__ __forceinline__ int seg_prefix_min(const int* __restrict__ in, const 
     if (s0 % TILE != 0) {
         const int c0 = s0 / CHUNK;
         if (s0 + lane < (c0 + 1) * CHUNK) m = in[s0 + lane];
-        const int c = c0 + 1 + lane;
-        if (c < (u + 1) * CPT) m = min(m, to.cmin[c]);
+        for (int c = c0 + 1 + lane; c < (u + 1) * CPT; c += WAVE) m = min(m, to.cmin[c]);
         a = u + 1;
     }
     return min(wave_min(m), range_min_tiles(L, a, t));

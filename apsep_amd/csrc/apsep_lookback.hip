@@ -176,7 +176,8 @@ __global__ __launch_bounds__(NT) void k_lookback(const int* __restrict__ in,
     int tmin = sm.wmin[0];
 #pragma unroll
     for (int w = 1; w < NWAVE; w++) tmin = min(tmin, sm.wmin[w]);
-    st_agent(&V.CM[(size_t)t * CPT + lane], tagged(V.tag, to.cmin[(size_t)t * CPT + lane]));
+    for (int c = lane; c < CPT; c += WAVE)
+        st_agent(&V.CM[(size_t)t * CPT + c], tagged(V.tag, to.cmin[(size_t)t * CPT + c]));
     if (lane == 0) st_agent(&V.M[t], tagged(V.tag, tmin));
 
     // 3. look back: exclusive prefix-min over tiles < t
