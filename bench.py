@@ -526,7 +526,9 @@ def main():
                       "level_build": round(levels_ms * 1e3, 2),
                       "fix_pass": round(fix_ms * 1e3, 2), "launches": int(runs),
                       "tile_pass_per_class": {k: round(t.tile_ms / t.runs * 1e3, 2)
-                                              for k, t in kt.items()}},
+                                              for k, t in kt.items()},
+                      "fix_pass_per_class": {k: round(t.fix_ms / t.runs * 1e3, 2)
+                                             for k, t in kt.items()}},
         "copy_gbs": round(copy_gbs(torch, n, stream), 1),
     }
     if not args.no_cpu:
