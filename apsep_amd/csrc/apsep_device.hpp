@@ -930,8 +930,10 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     const int sc2 = (t >= 2) ? to.scount[t - 2] : 0;
     const int cnt = to.qcount[t];
     const int v1 = in[tile_start];
-    int2 e = make_int2(0, INF);
-    if (lane < min(cnt, QS)) e = to.qlist[(size_t)t * QS + lane];
+    // the list slots are read with the rest of the first batch, before the
+    // count is known; slots past the count are ignored below
+    int2 e = to.qlist[(size_t)t * QS + (lane & (QS - 1))];
+    if (lane >= min(cnt, QS)) e = make_int2(0, INF);
     // exclusive prefix-min of tile t: level 0 from the window, upper levels
     // from the keys
     int p = (tw >= ((t >> LOG_FAN) << LOG_FAN)) ? mw : INF;
