@@ -934,10 +934,22 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     // count is known; slots past the count are ignored below
     int2 e = to.qlist[(size_t)t * QS + (lane & (QS - 1))];
     if (lane >= min(cnt, QS)) e = make_int2(0, INF);
+    // two levels (at most 64 groups of 64 tiles): every level-1 value and
+    // the tile group just before the window come with the first batch, so
+    // a search older than the window costs one more round, not three
+    const bool two = L.nlev == 2;
+    const int gs = ((t - WAVE - 1) >> LOG_FAN) << LOG_FAN;  // group of tile t - 65
+    int k1 = INF, mw2 = INF;
+    if (two) {
+        if (lane < L.n[1]) k1 = key_value(L.key[1][lane], L.epoch);
+        if (t > WAVE && gs + lane < t - WAVE) mw2 = L.mn0[gs + lane];
+    }
     // exclusive prefix-min of tile t: level 0 from the window, upper levels
     // from the keys
     int p = (tw >= ((t >> LOG_FAN) << LOG_FAN)) ? mw : INF;
-    if (L.nlev > 1) {
+    if (two) {
+        if (lane < (t >> LOG_FAN)) p = min(p, k1);
+    } else if (L.nlev > 1) {
         const int a1 = t >> LOG_FAN;
         const int b1 = (a1 >> LOG_FAN) << LOG_FAN;
         if (b1 + lane < a1) p = min(p, key_value(L.key[1][b1 + lane], L.epoch));
@@ -983,7 +995,24 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
         while (deep) {
             const int l = __ffsll(deep) - 1;
             deep &= deep - 1;
-            const int r = find_tile_wave<INCL>(L, t - WAVE, __shfl(v, l, WAVE));
+            const int vv = __shfl(v, l, WAVE);
+            int r = -1;
+            if (!two) {
+                r = find_tile_wave<INCL>(L, t - WAVE, vv);
+            } else if (t > WAVE) {
+                const u64 b0 = __ballot(gs + lane < t - WAVE && qual<INCL>(mw2, vv));
+                if (b0) {
+                    r = gs + 63 - __clzll(b0);
+                } else {
+                    const u64 b1 = __ballot(lane < (gs >> LOG_FAN) && qual<INCL>(k1, vv));
+                    if (b1) {
+                        const int g = 63 - __clzll(b1);
+                        const int idx = (g << LOG_FAN) + lane;
+                        const u64 b2 = __ballot(idx < L.n[0] && qual<INCL>(L.mn0[idx], vv));
+                        r = b2 ? (g << LOG_FAN) + 63 - __clzll(b2) : -1;
+                    }
+                }
+            }
             if (lane == l) ts = r;
         }
         if (active) {
