@@ -18,8 +18,11 @@ Printed JSON line (rank 0, one line):
                (profiles/traffic.json; null when not measured)
   cpu_baseline the oracle's monotone stack (1 thread) on a bounded sample of
                the same inputs, timed on this host
+  per_class_*  each class alone: `steps` solves back to back between one
+               event pair, after the timed region
 Timing: HIP events on the stream the kernels are launched on (torch's current
-stream, passed to the C ABI).
+stream, passed to the C ABI); the timed region holds the `steps` steps and no
+other event.
 """
 from __future__ import annotations
 
@@ -51,40 +54,38 @@ def make_inputs(torch, n: int, seed: int = 2024):
 
 
 def time_solves(torch, scratch, xs, outs, steps, warmup, stream):
-    """Mean per-launch time (ms) of each class, HIP events on `stream`."""
+    """Timed region: `steps` steps of one solve per class, back to back on
+    `stream` with nothing between the solves, one HIP event pair around the
+    whole region.  Then, outside it, the per-class breakdown: `steps` solves
+    of one class back to back between one event pair, and each kernel of the
+    solve launched `steps` times back to back per class."""
     names = list(xs)
     for _ in range(warmup):
         for k in names:
             scratch.run(xs[k], outs[k], stream=stream)
     torch.cuda.synchronize()
-    per = {k: [] for k in names}
     t_all0 = torch.cuda.Event(enable_timing=True)
     t_all1 = torch.cuda.Event(enable_timing=True)
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(steps * len(names))]
-    torch.cuda.synchronize()
     t_all0.record(stream)
-    j = 0
     for _ in range(steps):
         for k in names:
-            a, b = ev[j]
-            a.record(stream)
             scratch.run(xs[k], outs[k], stream=stream)
-            b.record(stream)
-            j += 1
     t_all1.record(stream)
     torch.cuda.synchronize()
     scratch.check()
-    # per-kernel device time: each kernel of the solve launched `steps` times
-    # back to back per class, one HIP event pair per batch on the same stream
+    per = {}
+    for k in names:
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(steps):
+            scratch.run(xs[k], outs[k], stream=stream)
+        b.record(stream)
+        torch.cuda.synchronize()
+        per[k] = a.elapsed_time(b) / steps
+    scratch.check()
     kt = {k: scratch.time_kernels(xs[k], outs[k], steps, stream=stream) for k in names}
     scratch.check()
-    j = 0
-    for _ in range(steps):
-        for k in names:
-            a, b = ev[j]
-            per[k].append(a.elapsed_time(b))
-            j += 1
     return per, t_all0.elapsed_time(t_all1), kt
 
 
@@ -476,7 +477,7 @@ def main():
     with apsep_amd.Scratch(args.algo, n) as s:
         per, total_ms, kt = time_solves(torch, s, xs, outs, args.steps, args.warmup, stream)
         algo_name = {v: k for k, v in apsep_amd.ALGOS.items()}[s.algo]
-    mean = {k: sum(v) / len(v) for k, v in per.items()}
+    mean = per
     useful = 8.0 * n
     per_class = {k: round(useful / (mean[k] * 1e-3) / 1e9, 1) for k in CLASSES}
     step_ms = total_ms / args.steps
