@@ -169,9 +169,9 @@ __device__ __forceinline__ int seg_rel(int res, int i, int base, const SegLane& 
 }
 
 __device__ __forceinline__ int tile_value(const TileSmem& sm, int e) {
-    const int4 w = sm.val4[(e >> 2) & (K4 - 1)][e >> 4];
-    const int c = e & 3;
-    return c == 0 ? w.x : c == 1 ? w.y : c == 2 ? w.z : w.w;
+    // one dword read: val4[(e >> 2) & 3][e >> 4], component e & 3
+    const int* w = reinterpret_cast<const int*>(&sm.val4[0][0]);
+    return w[((e >> 2) & (K4 - 1)) * (NT * 4) + (e >> 4) * 4 + (e & 3)];
 }
 
 __device__ __forceinline__ int wave_shfl_up(int x, int d) {
@@ -373,28 +373,38 @@ __device__ __forceinline__ int last_in_lane(const TileSmem& sm, int lt, int v) {
 template <bool INCL>
 __device__ __forceinline__ int intile_answer(const TileSmem& sm, int qt, int val) {
     const int wq = qt >> 6, ql = qt & (WAVE - 1), r = ql >> 4;
-    int p;
-    // min of the own row's lanes before qt: two 8-lane windows
+    // every read is unconditional (clamped index) and every choice a
+    // select: the queue resolves one query per lane, and a branch taken by
+    // any lane would be paid by all of them
     const int rl = ql & 15;
-    const int rowpre = rl == 0 ? INF
-                     : min(sm.win[3][qt - 1], rl > 8 ? sm.win[3][qt - 9] : INF);
-    if (rl > 0 && qual<INCL>(rowpre, val)) {
-        p = qt - 1;
-    } else {
-        int4 rm = sm.rmin[wq];
-        int rr = (r > 2 && qual<INCL>(rm.z, val)) ? 2
-               : (r > 1 && qual<INCL>(rm.y, val)) ? 1
-               : (r > 0 && qual<INCL>(rm.x, val)) ? 0 : -1;
-        int w = wq;
-        if (rr < 0) {
-            w = wq - 1;  // an earlier wave qualifies; bounded anyway
-            while (w > 0 && !qual<INCL>(sm.wmin[w], val)) w--;
-            rm = sm.rmin[w];
-            rr = qual<INCL>(rm.w, val) ? 3 : qual<INCL>(rm.z, val) ? 2
-               : qual<INCL>(rm.y, val) ? 1 : 0;
-        }
-        p = w * WAVE + rr * 16 + 15;
-    }
+    const int a1 = sm.win[3][max(qt - 1, 0)];
+    const int a9 = sm.win[3][max(qt - 9, 0)];
+    const int4 rq = sm.rmin[wq];
+    const int4 wa = *reinterpret_cast<const int4*>(&sm.wmin[0]);
+    const int4 wb = *reinterpret_cast<const int4*>(&sm.wmin[4]);
+    // min of the own row's lanes before qt: two 8-lane windows
+    const int rowpre = min(a1, rl > 8 ? a9 : INF);
+    // nearest earlier row of the own wave holding a qualifying element
+    int rr = -1;
+    rr = ((r > 0) & qual<INCL>(rq.x, val)) ? 0 : rr;
+    rr = ((r > 1) & qual<INCL>(rq.y, val)) ? 1 : rr;
+    rr = ((r > 2) & qual<INCL>(rq.z, val)) ? 2 : rr;
+    // nearest earlier wave holding one (wave 0 when none is found: the
+    // caller guarantees an answer in the tile)
+    int w = 0;
+    w = ((wq > 1) & qual<INCL>(wa.y, val)) ? 1 : w;
+    w = ((wq > 2) & qual<INCL>(wa.z, val)) ? 2 : w;
+    w = ((wq > 3) & qual<INCL>(wa.w, val)) ? 3 : w;
+    w = ((wq > 4) & qual<INCL>(wb.x, val)) ? 4 : w;
+    w = ((wq > 5) & qual<INCL>(wb.y, val)) ? 5 : w;
+    w = ((wq > 6) & qual<INCL>(wb.z, val)) ? 6 : w;
+    w = ((wq > 7) & qual<INCL>(wb.w, val)) ? 7 : w;
+    const int4 rw = sm.rmin[w];
+    const int rw_last = qual<INCL>(rw.w, val) ? 3 : qual<INCL>(rw.z, val) ? 2
+                      : qual<INCL>(rw.y, val) ? 1 : 0;
+    int p = ((rl > 0) & qual<INCL>(rowpre, val)) ? qt - 1
+          : rr >= 0 ? wq * WAVE + rr * 16 + 15
+          : w * WAVE + rw_last * 16 + 15;
     // nearest lane <= p of p's row whose min qualifies (one exists)
 #pragma unroll
     for (int s = 3; s >= 0; s--)
