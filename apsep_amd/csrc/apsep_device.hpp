@@ -435,6 +435,55 @@ __device__ __forceinline__ void load_lane(const int* __restrict__ in, int n, int
     }
 }
 
+// r = the last j in [J0, min(J0 + 4, i)) with qual(v[j], v[i]), else r
+// (j as an inline constant), for the blocks J0 = 0, 4, 8, 12 below i.  The
+// up-to-four compares of a block write four separate lane masks before the
+// selects read them: a select reading the mask its compare has just written
+// would wait two cycles (the hardware's VALU-writes-SGPR, VALU-reads-it-as-
+// mask rule), which the compiler pays with a nop on every one of the 120
+// pairs when it keeps the mask in VCC.
+template <bool INCL, int J0>
+__device__ __forceinline__ void inlane_select(int& r, const int* v, int i) {
+    if (J0 >= K - 1 || J0 >= i) return;
+    const int n = i - J0;  // compares in this block (>= 1)
+    u64 m0, m1, m2, m3;
+    if (n >= 4) {
+        if (INCL)
+            asm volatile(
+                "v_cmp_le_i32_e64 %[m0], %[a0], %[vi]\n"
+                "v_cmp_le_i32_e64 %[m1], %[a1], %[vi]\n"
+                "v_cmp_le_i32_e64 %[m2], %[a2], %[vi]\n"
+                "v_cmp_le_i32_e64 %[m3], %[a3], %[vi]\n"
+                "v_cndmask_b32_e64 %[r], %[r], %[j0], %[m0]\n"
+                "v_cndmask_b32_e64 %[r], %[r], %[j1], %[m1]\n"
+                "v_cndmask_b32_e64 %[r], %[r], %[j2], %[m2]\n"
+                "v_cndmask_b32_e64 %[r], %[r], %[j3], %[m3]\n"
+                : [r] "+v"(r), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3)
+                : [a0] "v"(v[J0]), [a1] "v"(v[J0 + 1]), [a2] "v"(v[J0 + 2]), [a3] "v"(v[J0 + 3]),
+                  [vi] "v"(v[i]), [j0] "I"(J0), [j1] "I"(J0 + 1), [j2] "I"(J0 + 2), [j3] "I"(J0 + 3));
+        else
+            asm volatile(
+                "v_cmp_lt_i32_e64 %[m0], %[a0], %[vi]\n"
+                "v_cmp_lt_i32_e64 %[m1], %[a1], %[vi]\n"
+                "v_cmp_lt_i32_e64 %[m2], %[a2], %[vi]\n"
+                "v_cmp_lt_i32_e64 %[m3], %[a3], %[vi]\n"
+                "v_cndmask_b32_e64 %[r], %[r], %[j0], %[m0]\n"
+                "v_cndmask_b32_e64 %[r], %[r], %[j1], %[m1]\n"
+                "v_cndmask_b32_e64 %[r], %[r], %[j2], %[m2]\n"
+                "v_cndmask_b32_e64 %[r], %[r], %[j3], %[m3]\n"
+                : [r] "+v"(r), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3)
+                : [a0] "v"(v[J0]), [a1] "v"(v[J0 + 1]), [a2] "v"(v[J0 + 2]), [a3] "v"(v[J0 + 3]),
+                  [vi] "v"(v[i]), [j0] "I"(J0), [j1] "I"(J0 + 1), [j2] "I"(J0 + 2), [j3] "I"(J0 + 3));
+        if constexpr (J0 + 4 < K - 1) inlane_select<INCL, J0 + 4>(r, v, i);
+    } else {
+        // the last 1..3 compares: plain selects (the compiler separates a
+        // lone compare from its select as needed)
+#pragma unroll
+        for (int j = J0; j < J0 + 3; j++)
+            if (j < i) r = qual<INCL>(v[j], v[i]) ? j : r;
+    }
+}
+
 template <bool INCL, bool SEG = false>
 __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, const int* v,
                                            TileSmem& sm, const TileOut& to, const Levels& L,
@@ -450,13 +499,17 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     const int nvalid = min(max(n - base, 0), K);
     const unsigned vmask = (nvalid >= K) ? 0xFFFFu : ((1u << nvalid) - 1u);
 
-    // 1. in-lane answers (global index, -1 if none in the lane)
+    // 1. in-lane answers (global index, -1 if none in the lane).  The lane
+    //    offset of the answer is selected among inline constants, starting
+    //    from `none` (base + none == -1), and rebased once.
     int res[K];
+    res[0] = -1;
+    const int none = -1 - base;
 #pragma unroll
-    for (int i = 0; i < K; i++) {
-        res[i] = -1;
-#pragma unroll
-        for (int j = 0; j < i; j++) res[i] = qual<INCL>(v[j], v[i]) ? base + j : res[i];
+    for (int i = 1; i < K; i++) {
+        int r = none;
+        inlane_select<INCL, 0>(r, v, i);
+        res[i] = base + r;
     }
     int lmin = v[0];
 #pragma unroll
