@@ -351,17 +351,18 @@ __device__ int find_tile(const Levels& L, int t, int v) {
 }
 
 // Rightmost j in [0, K) with qual(lane lt's value j, v), or -1.
+template <bool INCL, int J0>
+__device__ __forceinline__ void sel4(int& r, int a0, int a1, int a2, int a3, int x);
+
 template <bool INCL>
 __device__ __forceinline__ int last_in_lane(const TileSmem& sm, int lt, int v) {
     int r = -1;
-#pragma unroll
-    for (int q = 0; q < K4; q++) {
-        const int4 w = sm.val4[q][lt];
-        r = qual<INCL>(w.x, v) ? 4 * q + 0 : r;
-        r = qual<INCL>(w.y, v) ? 4 * q + 1 : r;
-        r = qual<INCL>(w.z, v) ? 4 * q + 2 : r;
-        r = qual<INCL>(w.w, v) ? 4 * q + 3 : r;
-    }
+    const int4 w0 = sm.val4[0][lt], w1 = sm.val4[1][lt];
+    const int4 w2 = sm.val4[2][lt], w3 = sm.val4[3][lt];
+    sel4<INCL, 0>(r, w0.x, w0.y, w0.z, w0.w, v);
+    sel4<INCL, 4>(r, w1.x, w1.y, w1.z, w1.w, v);
+    sel4<INCL, 8>(r, w2.x, w2.y, w2.z, w2.w, v);
+    sel4<INCL, 12>(r, w3.x, w3.y, w3.z, w3.w, v);
     return r;
 }
 
@@ -435,49 +436,51 @@ __device__ __forceinline__ void load_lane(const int* __restrict__ in, int n, int
     }
 }
 
-// r = the last j in [J0, min(J0 + 4, i)) with qual(v[j], v[i]), else r
-// (j as an inline constant), for the blocks J0 = 0, 4, 8, 12 below i.  The
-// up-to-four compares of a block write four separate lane masks before the
-// selects read them: a select reading the mask its compare has just written
-// would wait two cycles (the hardware's VALU-writes-SGPR, VALU-reads-it-as-
-// mask rule), which the compiler pays with a nop on every one of the 120
-// pairs when it keeps the mask in VCC.
+// r = J0 + k for the last k in 0..3 with qual(a_k, x), else r unchanged.
+// The four compares write four separate lane masks before the selects read
+// them: a select reading the mask its compare has just written waits two
+// cycles (the hardware's VALU-writes-SGPR, VALU-reads-it-as-mask rule), which
+// the compiler pays with a nop on every pair when it keeps the mask in VCC.
+template <bool INCL, int J0>
+__device__ __forceinline__ void sel4(int& r, int a0, int a1, int a2, int a3, int x) {
+    u64 m0, m1, m2, m3;
+    if (INCL)
+        asm volatile(
+            "v_cmp_le_i32_e64 %[m0], %[a0], %[x]\n"
+            "v_cmp_le_i32_e64 %[m1], %[a1], %[x]\n"
+            "v_cmp_le_i32_e64 %[m2], %[a2], %[x]\n"
+            "v_cmp_le_i32_e64 %[m3], %[a3], %[x]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j0], %[m0]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j1], %[m1]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j2], %[m2]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j3], %[m3]\n"
+            : [r] "+v"(r), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3)
+            : [a0] "v"(a0), [a1] "v"(a1), [a2] "v"(a2), [a3] "v"(a3), [x] "v"(x),
+              [j0] "I"(J0), [j1] "I"(J0 + 1), [j2] "I"(J0 + 2), [j3] "I"(J0 + 3));
+    else
+        asm volatile(
+            "v_cmp_lt_i32_e64 %[m0], %[a0], %[x]\n"
+            "v_cmp_lt_i32_e64 %[m1], %[a1], %[x]\n"
+            "v_cmp_lt_i32_e64 %[m2], %[a2], %[x]\n"
+            "v_cmp_lt_i32_e64 %[m3], %[a3], %[x]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j0], %[m0]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j1], %[m1]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j2], %[m2]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j3], %[m3]\n"
+            : [r] "+v"(r), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3)
+            : [a0] "v"(a0), [a1] "v"(a1), [a2] "v"(a2), [a3] "v"(a3), [x] "v"(x),
+              [j0] "I"(J0), [j1] "I"(J0 + 1), [j2] "I"(J0 + 2), [j3] "I"(J0 + 3));
+}
+
+// r = the last j in [J0, i) with qual(v[j], v[i]), else r (j as an inline
+// constant): blocks of four, then the last one to three as plain selects.
 template <bool INCL, int J0>
 __device__ __forceinline__ void inlane_select(int& r, const int* v, int i) {
-    if (J0 >= K - 1 || J0 >= i) return;
-    const int n = i - J0;  // compares in this block (>= 1)
-    u64 m0, m1, m2, m3;
-    if (n >= 4) {
-        if (INCL)
-            asm volatile(
-                "v_cmp_le_i32_e64 %[m0], %[a0], %[vi]\n"
-                "v_cmp_le_i32_e64 %[m1], %[a1], %[vi]\n"
-                "v_cmp_le_i32_e64 %[m2], %[a2], %[vi]\n"
-                "v_cmp_le_i32_e64 %[m3], %[a3], %[vi]\n"
-                "v_cndmask_b32_e64 %[r], %[r], %[j0], %[m0]\n"
-                "v_cndmask_b32_e64 %[r], %[r], %[j1], %[m1]\n"
-                "v_cndmask_b32_e64 %[r], %[r], %[j2], %[m2]\n"
-                "v_cndmask_b32_e64 %[r], %[r], %[j3], %[m3]\n"
-                : [r] "+v"(r), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3)
-                : [a0] "v"(v[J0]), [a1] "v"(v[J0 + 1]), [a2] "v"(v[J0 + 2]), [a3] "v"(v[J0 + 3]),
-                  [vi] "v"(v[i]), [j0] "I"(J0), [j1] "I"(J0 + 1), [j2] "I"(J0 + 2), [j3] "I"(J0 + 3));
-        else
-            asm volatile(
-                "v_cmp_lt_i32_e64 %[m0], %[a0], %[vi]\n"
-                "v_cmp_lt_i32_e64 %[m1], %[a1], %[vi]\n"
-                "v_cmp_lt_i32_e64 %[m2], %[a2], %[vi]\n"
-                "v_cmp_lt_i32_e64 %[m3], %[a3], %[vi]\n"
-                "v_cndmask_b32_e64 %[r], %[r], %[j0], %[m0]\n"
-                "v_cndmask_b32_e64 %[r], %[r], %[j1], %[m1]\n"
-                "v_cndmask_b32_e64 %[r], %[r], %[j2], %[m2]\n"
-                "v_cndmask_b32_e64 %[r], %[r], %[j3], %[m3]\n"
-                : [r] "+v"(r), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3)
-                : [a0] "v"(v[J0]), [a1] "v"(v[J0 + 1]), [a2] "v"(v[J0 + 2]), [a3] "v"(v[J0 + 3]),
-                  [vi] "v"(v[i]), [j0] "I"(J0), [j1] "I"(J0 + 1), [j2] "I"(J0 + 2), [j3] "I"(J0 + 3));
+    if (J0 >= i) return;
+    if (i - J0 >= 4) {
+        sel4<INCL, J0>(r, v[J0], v[J0 + 1], v[J0 + 2], v[J0 + 3], v[i]);
         if constexpr (J0 + 4 < K - 1) inlane_select<INCL, J0 + 4>(r, v, i);
     } else {
-        // the last 1..3 compares: plain selects (the compiler separates a
-        // lone compare from its select as needed)
 #pragma unroll
         for (int j = J0; j < J0 + 3; j++)
             if (j < i) r = qual<INCL>(v[j], v[i]) ? j : r;
