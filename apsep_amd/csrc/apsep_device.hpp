@@ -868,15 +868,30 @@ struct LbCtl {
 };
 constexpr int LB_LEVELS = MAX_LEVELS;
 
-constexpr int FIX_LDS_INTS = WAVE + 4 * RS;  // tile-min window + 2 spines
+// per fix-up wave: tile-min window (64), its 16 four-tile group mins, two
+// spines (2 x RS int2)
+constexpr int FIX_LDS_INTS = WAVE + WAVE / 4 + 4 * RS;
 
 template <bool INCL>
 __device__ __forceinline__ int spine_answer_lds(const int2* sp, int cnt, int v) {
-    // lowest rank whose value qualifies (values decrease with rank)
+    // lowest rank whose value qualifies (values decrease with rank): batches
+    // of 16 ranks read at once and selected, highest rank first
     const int m = min(cnt, RS);
-    for (int r = 0; r < m; r++)
-        if (qual<INCL>(sp[r].y, v)) return sp[r].x;
-    return -1;
+    const int4* p = reinterpret_cast<const int4*>(sp);
+    int pos = -1;
+#pragma unroll
+    for (int h = 0; h < 2 && pos < 0 && h * (RS / 2) < m; h++) {
+        int4 w[RS / 4];
+#pragma unroll
+        for (int k = 0; k < RS / 4; k++) w[k] = p[h * (RS / 4) + k];
+#pragma unroll
+        for (int k = RS / 4 - 1; k >= 0; k--) {
+            const int r = h * (RS / 2) + 2 * k;
+            pos = (r + 1 < m && qual<INCL>(w[k].w, v)) ? w[k].z : pos;
+            pos = (r < m && qual<INCL>(w[k].y, v)) ? w[k].x : pos;
+        }
+    }
+    return pos;
 }
 
 template <bool INCL>
@@ -1035,8 +1050,14 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     const int P = SEG ? seg_prefix_min(in, to, L, s0, t) : wave_min(p);
     if (!qual<INCL>(P, v1)) return;  // every open element answers -1
     int* win = lds;
-    int2* sp = reinterpret_cast<int2*>(lds + WAVE);
+    int* gm = lds + WAVE;  // min of window tiles 4g .. 4g+3
+    int2* sp = reinterpret_cast<int2*>(lds + WAVE + WAVE / 4);
     win[lane] = mw;
+    {
+        int g4 = min(mw, dpp<DPP_ROW_SHR + 1>(INF, mw));
+        g4 = min(g4, dpp<DPP_ROW_SHR + 2>(INF, g4));
+        if ((lane & 3) == 3) gm[lane >> 2] = g4;
+    }
     if (lane < 2 * RS) sp[lane] = sw;
     wave_lds_sync();
     if (cnt <= QS) {
@@ -1044,12 +1065,27 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
         const int v = e.y;
         // tile: the window first (tiles t-64 .. t-1) ...
         int ts = -1;
+        const int m_prev = readlane(mw, WAVE - 1);  // min of tile t - 1
         if (active) {
-            int s = -1;
-            for (int i = WAVE - 4; i >= 0 && s < 0; i -= 4) {
-                const int4 w = *reinterpret_cast<const int4*>(&win[i]);
-                s = qual<INCL>(w.w, v) ? i + 3 : qual<INCL>(w.z, v) ? i + 2
-                  : qual<INCL>(w.y, v) ? i + 1 : qual<INCL>(w.x, v) ? i : -1;
+            int s = WAVE - 1;
+            if (!qual<INCL>(m_prev, v)) {
+                // the last qualifying group of four window tiles, then the
+                // last qualifying tile in it: two LDS rounds, whatever the
+                // distance
+                const int4* g4 = reinterpret_cast<const int4*>(gm);
+                const int4 ga = g4[0], gb = g4[1], gc = g4[2], gd = g4[3];
+                int g = -1;
+                sel4<INCL, 0>(g, ga.x, ga.y, ga.z, ga.w, v);
+                sel4<INCL, 4>(g, gb.x, gb.y, gb.z, gb.w, v);
+                sel4<INCL, 8>(g, gc.x, gc.y, gc.z, gc.w, v);
+                sel4<INCL, 12>(g, gd.x, gd.y, gd.z, gd.w, v);
+                s = -1;
+                if (g >= 0) {
+                    const int4 w = reinterpret_cast<const int4*>(win)[g];
+                    int k = -1;
+                    sel4<INCL, 0>(k, w.x, w.y, w.z, w.w, v);
+                    s = 4 * g + k;  // k >= 0: the group's min qualifies
+                }
             }
             // window entries of tiles < 0 hold INF, which never qualifies
             // except under INCL with v == INF: exclude them explicitly
