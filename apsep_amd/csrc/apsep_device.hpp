@@ -487,13 +487,27 @@ __device__ __forceinline__ void inlane_select(int& r, const int* v, int i) {
     }
 }
 
+// Bit i = qual(p, v[i]), from the sign of a saturating difference (no lane
+// masks): p < v iff sat(p - v) < 0; p <= v iff not sat(v - p) < 0.
+template <bool INCL>
+__device__ __forceinline__ unsigned qual_bits(int p, const int* v) {
+    unsigned m = 0;
+#pragma unroll
+    for (int i = K - 1; i >= 0; i--) {
+        const int d = INCL ? __builtin_elementwise_sub_sat(v[i], p)
+                           : __builtin_elementwise_sub_sat(p, v[i]);
+        m = __builtin_amdgcn_alignbit(m, (unsigned)d, 31);
+    }
+    return INCL ? (~m & 0xFFFFu) : m;
+}
+
 template <bool INCL, bool SEG = false>
 __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, const int* v,
                                            TileSmem& sm, const TileOut& to, const Levels& L,
                                            const SegLane sl = SegLane{0, 0u}) {
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar tests
     const int row = lane >> 4;
     const int tile_start = t * TILE;
     const int base = tile_start + tid * K;
@@ -566,16 +580,15 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     // (non-increasing, the last one is the lane min), so when pmx beats the
     // lane min it beats them all and every one has an answer in the tile;
     // only lanes holding tile prefix minima need the per-element split.
+    // (masks are assembled one bit per op, highest element first:
+    // alignbit(m, x, 31) = m << 1 | the sign bit of x)
     unsigned rec = 0;
 #pragma unroll
-    for (int i = 0; i < K; i++) rec |= ((unsigned)res[i] >> 31) << i;
+    for (int i = K - 1; i >= 0; i--) rec = __builtin_amdgcn_alignbit(rec, (unsigned)res[i], 31);
     unsigned qmask = rec, omask = 0;
     if (!has_prev || !qual<INCL>(pmx, lmin)) {
         unsigned beaten = 0;
-        if (has_prev) {
-#pragma unroll
-            for (int i = 0; i < K; i++) beaten |= (qual<INCL>(pmx, v[i]) ? 1u : 0u) << i;
-        }
+        if (has_prev) beaten = qual_bits<INCL>(pmx, v);
         qmask = rec & beaten;
         omask = rec & ~beaten;
     }
