@@ -194,8 +194,10 @@ apsep_status apsep_batch_plan_create(int64_t total_cap, int32_t nseg_cap, int in
         return st;
     }
     const int64_t tiles = (total_cap + TILE - 1) / TILE;
-    if (tiles > 0 && hipMalloc((void**)&p->tmap, (size_t)tiles * sizeof(int)) != hipSuccess) {
+    if (tiles > 0 && (hipMalloc((void**)&p->tmap, (size_t)tiles * sizeof(int)) != hipSuccess ||
+                      hipMalloc((void**)&p->tbound, (size_t)tiles * sizeof(int2)) != hipSuccess)) {
         (void)hipGetLastError();
+        if (p->tmap) (void)hipFree(p->tmap);
         apsep_scratch_destroy(p->s);
         delete p;
         return APSEP_ERR_ALLOC;
@@ -215,6 +217,7 @@ apsep_status apsep_batch_plan_destroy(apsep_batch_plan* p) {
     if (!p) return APSEP_OK;
     drop_graph(p);
     if (p->tmap) (void)hipFree(p->tmap);
+    if (p->tbound) (void)hipFree(p->tbound);
     apsep_scratch_destroy(p->s);
     delete p;
     return APSEP_OK;

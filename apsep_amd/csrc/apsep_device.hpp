@@ -149,6 +149,7 @@ struct SegIn {
     const long long* offs;  // [nseg + 1], non-decreasing, offs[0] = 0, offs[nseg] = n
     const int* tmap;        // [nt]: segment holding each tile's first element
     int nseg;
+    const int2* tbound;     // [nt]: that segment's start and end
 };
 
 // Per lane of a segmented tile: the start of the segment holding the lane's
@@ -213,10 +214,16 @@ __device__ __forceinline__ int wave_incl_add(int x) {
     return x;
 }
 
+// Min over the wave (DPP row scans, then the row carries; no LDS round
+// trip).  The whole wave must be active.
 __device__ __forceinline__ int wave_min(int x) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) x = min(x, __shfl_xor(x, d, WAVE));
-    return x;
+    x = min(x, dpp<DPP_ROW_SHR + 1>(INF, x));
+    x = min(x, dpp<DPP_ROW_SHR + 2>(INF, x));
+    x = min(x, dpp<DPP_ROW_SHR + 4>(INF, x));
+    x = min(x, dpp<DPP_ROW_SHR + 8>(INF, x));
+    x = min(x, dpp<DPP_ROW_BCAST15, 0xa>(INF, x));
+    x = min(x, dpp<DPP_ROW_BCAST31, 0xc>(INF, x));
+    return lane63(x);
 }
 
 // Intra-wave LDS hand-off: LDS operations of one wave complete in order; this
@@ -289,37 +296,8 @@ __device__ __forceinline__ int last_in_64(const int* __restrict__ arr, int b, in
     return r;
 }
 
-// Exclusive prefix-min of tile t over tiles [0, t) (t >= 1); called by a
-// whole wave (uniform t), result uniform.  Per level, the entries before
-// t's ancestor inside the ancestor's 64-group (one coalesced wave load per
-// level, all independent); without stored upper levels, all level-1
-// entries before the ancestor's 64-group (at most UPPER_MIN / 64 loads).
-__device__ __forceinline__ int tile_prefix(const Levels& L, int t) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int g0 = (t >> LOG_FAN) << LOG_FAN;
-    int p = (g0 + lane < t) ? L.mn0[g0 + lane] : INF;
-    if (L.nlev > 1) {
-        const int a1 = t >> LOG_FAN;
-        const int b1 = (a1 >> LOG_FAN) << LOG_FAN;
-        if (b1 + lane < a1) p = min(p, key_value(L.key[1][b1 + lane], L.epoch));
-        if (L.upper) {
-#pragma unroll
-            for (int l = 2; l < MAX_LEVELS; l++) {
-                if (l < L.nlev) {
-                    const int a = t >> (LOG_FAN * l);
-                    const int idx = ((a >> LOG_FAN) << LOG_FAN) + lane;
-                    if (idx < a) p = min(p, key_value(L.key[l][idx], L.epoch));
-                }
-            }
-        } else {
-            for (int i = lane; i < b1; i += WAVE) p = min(p, key_value(L.key[1][i], L.epoch));
-        }
-    }
-    return wave_min(p);
-}
-
 // Nearest tile s < t with qual(tile min, v); the caller guarantees one
-// exists (qual(tile_prefix(t), v)).  Ascends the 64-ary hierarchy until an
+// exists (some tile before t qualifies).  Ascends the 64-ary hierarchy until an
 // earlier entry qualifies, then descends to the tile.
 template <bool INCL>
 __device__ int find_tile(const Levels& L, int t, int v) {
@@ -772,7 +750,7 @@ __device__ __forceinline__ void tile_step_seg(const int* __restrict__ in, int* _
     const int last = max(sg.nseg - 1, 0);
     const int k0 = min(max(sg.tmap[t], 0), last);
     const int k1 = (t + 1 < L.nt) ? min(max(sg.tmap[t + 1], k0), last) : last;
-    const int s0 = (int)min(max(sg.offs[k0], 0ll), (long long)tile_start);
+    const int s0 = min(max(sg.tbound[t].x, 0), tile_start);
     if (tid < TILE / 32) sm.walls[tid] = 0u;
     __syncthreads();
     for (int k = k0 + 1 + tid; k <= k1; k += NT) {
@@ -955,65 +933,82 @@ __device__ __forceinline__ int answer_in_tile(const int* __restrict__ in, const 
     return e >= 0 ? e : last_in_32<INCL>(in, c * CHUNK, v);
 }
 
-// Min over the tiles [a, b) (whole wave, uniform result): partial 64-groups
-// at both ends on each level, full groups one level up.
-__device__ __forceinline__ int range_min_tiles(const Levels& L, int a, int b) {
+// This lane's share of the min over the tiles [a, b) (whole wave, uniform a
+// and b; the wave min of the shares is the range min): partial 64-groups at
+// both ends on each level, full groups one level up.  The ranges of all
+// levels follow from a and b alone, so every level's loads are issued
+// before any is used (one round, not one per level).
+__device__ __forceinline__ int range_min_lane(const Levels& L, int a, int b) {
     const int lane = threadIdx.x & (WAVE - 1);
-    int m = INF;
-    int lvl = 0;
-    while (a < b) {
+    // raw entries: level 0 as loaded, keys (levels >= 1) converted after
+    // every load is issued (0 reads as INF: a key of no run)
+    int e0a = INF, e0b = INF;
+    u64 ek[MAX_LEVELS][2];
+    int mtop = INF;
+    bool done = false;
+#pragma unroll
+    for (int lvl = 0; lvl < MAX_LEVELS; lvl++) {
+        ek[lvl][0] = ek[lvl][1] = 0;
+        if (done || a >= b) {
+            done = true;
+            continue;
+        }
         const bool up_stored = lvl + 1 < L.nlev && (lvl + 1 < 2 || L.upper);
         if (!up_stored) {
-            for (int i = a + lane; i < b; i += WAVE) m = min(m, level_entry(L, lvl, i));
-            break;
+            // the top stored level: the whole rest of the range
+            for (int i = a + lane; i < b; i += WAVE) mtop = min(mtop, level_entry(L, lvl, i));
+            done = true;
+            continue;
         }
         const int ga = min(((a + FAN - 1) >> LOG_FAN) << LOG_FAN, b);
         const int gb = max((b >> LOG_FAN) << LOG_FAN, ga);
-        if (a + lane < ga) m = min(m, level_entry(L, lvl, a + lane));
-        if (gb + lane < b) m = min(m, level_entry(L, lvl, gb + lane));
+        if (lvl == 0) {
+            if (a + lane < ga) e0a = L.mn0[a + lane];
+            if (gb + lane < b) e0b = L.mn0[gb + lane];
+        } else {
+            if (a + lane < ga) ek[lvl][0] = L.key[lvl][a + lane];
+            if (gb + lane < b) ek[lvl][1] = L.key[lvl][gb + lane];
+        }
         a = ga >> LOG_FAN;
         b = gb >> LOG_FAN;
-        lvl++;
     }
-    return wave_min(m);
+    int m = min(mtop, min(e0a, e0b));
+#pragma unroll
+    for (int lvl = 1; lvl < MAX_LEVELS; lvl++)
+        m = min(m, min(key_value(ek[lvl][0], L.epoch), key_value(ek[lvl][1], L.epoch)));
+    return m;
 }
 
 // Min of in[s0 .. t * TILE) for s0 < t * TILE (whole wave, uniform result):
 // the rest of s0's chunk, the rest of its tile's chunk mins, then whole
 // tiles through the hierarchy.  Tile s0 / TILE precedes t, so it is complete.
+static_assert(CPT <= 2 * WAVE, "the rest of a tile's chunk mins takes two loads per lane");
 __device__ __forceinline__ int seg_prefix_min(const int* __restrict__ in, const TileOut& to,
                                               const Levels& L, int s0, int t) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int u = s0 / TILE;
-    int m = INF;
-    int a = u;
-    if (s0 % TILE != 0) {
+    const bool part = s0 % TILE != 0;
+    // all loads first, one wave min at the end
+    const int m3 = range_min_lane(L, part ? u + 1 : u, t);
+    int m0 = INF, m1 = INF, m2 = INF;
+    if (part) {
         const int c0 = s0 / CHUNK;
-        if (s0 + lane < (c0 + 1) * CHUNK) m = in[s0 + lane];
-        for (int c = c0 + 1 + lane; c < (u + 1) * CPT; c += WAVE) m = min(m, to.cmin[c]);
-        a = u + 1;
+        const int cend = (u + 1) * CPT;
+        if (s0 + lane < (c0 + 1) * CHUNK) m0 = in[s0 + lane];
+        if (c0 + 1 + lane < cend) m1 = to.cmin[c0 + 1 + lane];
+        if (c0 + 1 + WAVE + lane < cend) m2 = to.cmin[c0 + 1 + WAVE + lane];
     }
-    return min(wave_min(m), range_min_tiles(L, a, t));
+    return wave_min(min(min(m0, m1), min(m2, m3)));
 }
 
 template <bool INCL, bool SEG = false>
 __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __restrict__ out,
                                          int n, int t, const Levels& L, const TileOut& to,
-                                         int* lds, const SegIn sg = SegIn{nullptr, nullptr, 0}) {
+                                         int* lds, const SegIn sg = SegIn{nullptr, nullptr, 0, nullptr}) {
     if (t == 0) return;
     const int lane = threadIdx.x & (WAVE - 1);
     const int tile_start = t * TILE;
-    // segmented: only the open elements of the segment that runs into the
-    // tile from before (tile-relative positions < lim) can have an answer
-    int s0 = 0, lim = TILE;
-    if (SEG) {
-        const int last = max(sg.nseg - 1, 0);
-        const int k0 = min(max(sg.tmap[t], 0), last);
-        s0 = (int)min(max(sg.offs[k0], 0ll), (long long)tile_start);
-        if (s0 >= tile_start) return;
-        lim = (int)(min(max(sg.offs[k0 + 1], (long long)tile_start + 1),
-                        (long long)tile_start + TILE) - tile_start);
-    }
+    const int2 tb = SEG ? sg.tbound[t] : make_int2(0, 0);
     const int tw = t - WAVE + lane;  // tile whose min this lane holds
     const int mw = tw >= 0 ? L.mn0[tw] : INF;
     // spines of t-1 (lanes 0..RS-1) and t-2 (lanes RS..2RS-1)
@@ -1033,10 +1028,21 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     // a search older than the window costs one more round, not three
     const bool two = L.nlev == 2;
     const int gs = ((t - WAVE - 1) >> LOG_FAN) << LOG_FAN;  // group of tile t - 65
-    int k1 = INF, mw2 = INF;
+    u64 k1raw = 0;  // (a key of no run: reads as INF)
+    int mw2 = INF;
     if (two) {
-        if (lane < L.n[1]) k1 = key_value(L.key[1][lane], L.epoch);
+        if (lane < L.n[1]) k1raw = L.key[1][lane];
         if (t > WAVE && gs + lane < t - WAVE) mw2 = L.mn0[gs + lane];
+    }
+    const int k1 = key_value(k1raw, L.epoch);
+    // segmented: only the open elements of the segment that runs into the
+    // tile from before (tile-relative positions < lim) can have an answer
+    int s0 = 0, lim = TILE;
+    if (SEG) {
+        s0 = min(max(tb.x, 0), tile_start);
+        lim = (int)(min(max((long long)tb.y, (long long)tile_start + 1),
+                        (long long)tile_start + TILE) - tile_start);
+        if (s0 >= tile_start) return;
     }
     // exclusive prefix-min of tile t: level 0 from the window, upper levels
     // from the keys

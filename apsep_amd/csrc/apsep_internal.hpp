@@ -42,6 +42,7 @@ struct apsep_batch_plan {
     int64_t total_cap;
     int32_t nseg_cap;
     int* tmap;             // [tiles at capacity]
+    int2* tbound;          // [tiles at capacity]: that segment's start and end
     int use_graph;
     hipGraph_t graph;
     hipGraphExec_t exec;

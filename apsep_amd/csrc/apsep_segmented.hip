@@ -28,9 +28,11 @@ namespace apsep {
 constexpr unsigned SEG_EPOCH = 1u;
 constexpr int SEG_FIX_WG = 1024;
 
-// tile -> segment holding the tile's first element; level-1 keys emptied
+// tile -> segment holding the tile's first element, and that segment's
+// start and end; level-1 keys emptied
 __global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__ offs,
-                                                   int nseg, int* __restrict__ tmap, int nt,
+                                                   int nseg, int* __restrict__ tmap,
+                                                   int2* __restrict__ tbound, int nt,
                                                    u64* __restrict__ key1, int nkey1) {
     const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int stride = (int)(gridDim.x * blockDim.x);
@@ -42,7 +44,10 @@ __global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__
         if (s >= e) continue;
         const int t0 = (int)((s + TILE - 1) / TILE);
         const int t1 = (int)((e - 1) / TILE);
-        for (int t = t0; t <= t1; t++) tmap[t] = k;
+        for (int t = t0; t <= t1; t++) {
+            tmap[t] = k;
+            tbound[t] = make_int2((int)min(s, (long long)INT_MAX), (int)min(e, (long long)INT_MAX));
+        }
     }
 }
 
@@ -83,14 +88,14 @@ static apsep_status seg_launch(apsep_batch_plan* p, const int* in, const long lo
     }
     L.epoch = SEG_EPOCH;
     TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
-    SegIn sg{offs, p->tmap, nseg};
+    SegIn sg{offs, p->tmap, nseg, p->tbound};
     const int nkey1 = L.nlev > 1 ? sz[1] : 0;
     const int work = nseg > nkey1 ? nseg : nkey1;
     int blocks = (work + 255) / 256;
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_seg_setup, dim3(blocks), dim3(256), 0, st, offs, nseg, p->tmap, nt,
-                       nkey1 ? s->key[1] : nullptr, nkey1);
+    hipLaunchKernelGGL(k_seg_setup, dim3(blocks), dim3(256), 0, st, offs, nseg, p->tmap,
+                       p->tbound, nt, nkey1 ? s->key[1] : nullptr, nkey1);
     hipLaunchKernelGGL(k_tile_pass_seg<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L, sg);
     if (nt > 1) {
         if (L.upper) hipLaunchKernelGGL(k_upper_levels_seg, dim3(1), dim3(256), 0, st, L);
