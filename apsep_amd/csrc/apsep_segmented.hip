@@ -26,7 +26,7 @@
 namespace apsep {
 
 constexpr unsigned SEG_EPOCH = 1u;
-constexpr int SEG_FIX_WG = 1024;
+constexpr int SEG_FIX_WG = 256;
 
 // tile -> segment holding the tile's first element, and that segment's
 // start and end; level-1 keys emptied
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(NT) void k_tile_pass_seg(const int* __restrict__ in
 __global__ __launch_bounds__(256) void k_upper_levels_seg(Levels L) { build_upper_levels(L); }
 
 template <bool INCL>
-__global__ __launch_bounds__(SEG_FIX_WG) void k_fix_pass_seg(const int* __restrict__ in,
+__global__ __launch_bounds__(SEG_FIX_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_fix_pass_seg(const int* __restrict__ in,
                                                              int* __restrict__ out, int n,
                                                              Levels L, TileOut to, SegIn sg) {
     __shared__ int lds[SEG_FIX_WG / WAVE][FIX_LDS_INTS];
