@@ -37,10 +37,10 @@ __global__ __launch_bounds__(256) void k_upper_levels(Levels L) { build_upper_le
 
 // WSTL pass 3: one wave per tile.
 // (1024-lane workgroups: 16 tiles per workgroup, fewer dispatches)
-constexpr int FIX_WG = 1024;
+constexpr int FIX_WG = 256;
 
 template <bool INCL>
-__global__ __launch_bounds__(FIX_WG) void k_fix_pass(const int* __restrict__ in,
+__global__ __launch_bounds__(FIX_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_fix_pass(const int* __restrict__ in,
                                                      int* __restrict__ out, int n,
                                                      Levels L, TileOut to) {
     __shared__ int lds[FIX_WG / WAVE][FIX_LDS_INTS];
