@@ -26,7 +26,7 @@
 namespace apsep {
 
 constexpr unsigned SEG_EPOCH = 1u;
-constexpr int SEG_FIX_WG = 256;
+constexpr int SEG_FIX_WG = 256;  // four tiles per group, up to 8 waves per SIMD (as k_fix_pass)
 
 // tile -> segment holding the tile's first element, and that segment's
 // start and end; level-1 keys emptied

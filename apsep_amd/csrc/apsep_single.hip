@@ -36,7 +36,9 @@ __global__ __launch_bounds__(NT) void k_tile_pass(const int* __restrict__ in,
 __global__ __launch_bounds__(256) void k_upper_levels(Levels L) { build_upper_levels(L); }
 
 // WSTL pass 3: one wave per tile.
-// (1024-lane workgroups: 16 tiles per workgroup, fewer dispatches)
+// (256-lane workgroups of four tiles, held to 64 VGPRs so that 8 waves fit
+// per SIMD: the pass is latency-bound, and more tiles in flight beat a few
+// spilled registers)
 constexpr int FIX_WG = 256;
 
 template <bool INCL>
