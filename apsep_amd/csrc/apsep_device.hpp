@@ -602,21 +602,6 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         }
     }
 
-    // write the tile's outputs (answers found so far; -1 elsewhere)
-    if (nvalid == K) {
-#pragma unroll
-        for (int q = 0; q < K4; q++)
-            *reinterpret_cast<int4*>(out + base + 4 * q) =
-                make_int4(seg_rel<SEG>(res[4 * q], 4 * q, base, sl),
-                          seg_rel<SEG>(res[4 * q + 1], 4 * q + 1, base, sl),
-                          seg_rel<SEG>(res[4 * q + 2], 4 * q + 2, base, sl),
-                          seg_rel<SEG>(res[4 * q + 3], 4 * q + 3, base, sl));
-    } else {
-#pragma unroll
-        for (int i = 0; i < K; i++)
-            if (i < nvalid) out[base + i] = seg_rel<SEG>(res[i], i, base, sl);
-    }
-
     // one scan for both counts: queries (low 16 bits), open (high 16 bits)
     const int cq = __builtin_popcount(qmask), co = __builtin_popcount(omask);
     int packed = cq | (co << 16);
@@ -645,10 +630,31 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     }
     const int safter_w = zs - cs;
 
+    // The outputs of a wave whose queries all fit its queue (nearly every
+    // wave) are written after the queue, once, through LDS and coalesced
+    // (below).  A wave with more queries writes its own lanes' 16
+    // consecutive outputs now and patches the answers in afterwards.
+    const bool staged = qtotal <= QCAP;
+    if (!staged) {
+        if (nvalid == K) {
+#pragma unroll
+            for (int q = 0; q < K4; q++)
+                *reinterpret_cast<int4*>(out + base + 4 * q) =
+                    make_int4(seg_rel<SEG>(res[4 * q], 4 * q, base, sl),
+                              seg_rel<SEG>(res[4 * q + 1], 4 * q + 1, base, sl),
+                              seg_rel<SEG>(res[4 * q + 2], 4 * q + 2, base, sl),
+                              seg_rel<SEG>(res[4 * q + 3], 4 * q + 3, base, sl));
+        } else {
+#pragma unroll
+            for (int i = 0; i < K; i++)
+                if (i < nvalid) out[base + i] = seg_rel<SEG>(res[i], i, base, sl);
+        }
+    }
+
     // 3b. the rest through the wave's queue: compact, resolve one query per
-    //     lane, hand the answers back; the owner overwrites its placeholder
-    //     (same-thread store order).  Entries beyond QCAP are resolved by
-    //     their owner.
+    //     lane, hand the answers back.  In an unstaged wave the owner
+    //     overwrites its placeholder (same-thread store order); entries
+    //     beyond QCAP are resolved by their owner.
     if (qtotal > 0) {
         unsigned m = qmask;
         int k = qoff;
@@ -664,19 +670,64 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
             sm.q[wave][r] = (unsigned short)intile_answer<INCL>(sm, item >> 4, tile_value(sm, item));
         }
         wave_lds_sync();
-        m = qmask;
-        k = qoff;
+        if (!staged) {
+            m = qmask;
+            k = qoff;
+            while (m) {
+                const int i = __builtin_ctz(m);
+                m &= m - 1;
+                const int r = tile_start + ((k < QCAP) ? (int)sm.q[wave][k]
+                                                       : intile_answer<INCL>(sm, tid,
+                                                                             tile_value(sm, tid * K + i)));
+                out[base + i] = seg_rel<SEG>(r, i, base, sl);
+                k++;
+            }
+        }
+    }
+    __syncthreads();  // no wave reads the tile's values past this point
+
+    // staged outputs.  Each wave stages its 1024 outputs in its own lanes'
+    // value slots: 16-byte group g of the wave (elements 4g .. 4g+3) goes to
+    // val4[g & 3][wave * 64 + ((g >> 2) ^ ((g & 3) << 2))].  The owner writes
+    // its groups g = 4 lane + q, the queue answers are patched in, then lane
+    // l reads back groups g = 64 q + l and stores them: 1 KB contiguous per
+    // store instruction instead of 16 bytes at a 64-byte stride.  (The XOR
+    // keeps both the writes and the reads free of LDS bank conflicts.)
+    if (staged) {
+        int* sv = reinterpret_cast<int*>(&sm.val4[0][0]);
+#pragma unroll
+        for (int q = 0; q < K4; q++)
+            sm.val4[q][wave * WAVE + (lane ^ (q << 2))] =
+                make_int4(seg_rel<SEG>(res[4 * q], 4 * q, base, sl),
+                          seg_rel<SEG>(res[4 * q + 1], 4 * q + 1, base, sl),
+                          seg_rel<SEG>(res[4 * q + 2], 4 * q + 2, base, sl),
+                          seg_rel<SEG>(res[4 * q + 3], 4 * q + 3, base, sl));
+        unsigned m = qmask;
+        int k = qoff;
         while (m) {
             const int i = __builtin_ctz(m);
             m &= m - 1;
-            const int r = tile_start + ((k < QCAP) ? (int)sm.q[wave][k]
-                                                   : intile_answer<INCL>(sm, tid,
-                                                                         tile_value(sm, tid * K + i)));
-            out[base + i] = seg_rel<SEG>(r, i, base, sl);
+            const int q = i >> 2;
+            sv[(q * NT + wave * WAVE + (lane ^ (q << 2))) * 4 + (i & 3)] =
+                seg_rel<SEG>(tile_start + (int)sm.q[wave][k], i, base, sl);
             k++;
         }
+        wave_lds_sync();
+        const int wbase = tile_start + wave * (WAVE * K);
+#pragma unroll
+        for (int q = 0; q < K4; q++) {
+            const int c = lane & 3, r = 16 * q + (lane >> 2);
+            const int4 x = sm.val4[c][wave * WAVE + (r ^ (c << 2))];
+            const int e = wbase + q * (WAVE * 4) + lane * 4;
+            if (e + 3 < n) {
+                *reinterpret_cast<int4*>(out + e) = x;
+            } else {
+                if (e < n) out[e] = x.x;
+                if (e + 1 < n) out[e + 1] = x.y;
+                if (e + 2 < n) out[e + 2] = x.z;
+            }
+        }
     }
-    __syncthreads();
 
     // 4. open elements: (position, value) pairs in position order, up to QS
     int obase = ooff;

@@ -220,7 +220,7 @@ __global__ __launch_bounds__(NT) void k_lookback(const int* __restrict__ in,
 
     // 4. open elements that P beats (same rules as the multi-launch fix-up)
     if (t == 0) return;
-    const int v1 = tile_value(sm, 0);
+    const int v1 = __shfl(v[0], 0, WAVE);  // the tile's first element (lane 0's first value)
     if (!qual<INCL>(P, v1)) return;
     const int cnt = to.qcount[t];
     const int tw = t - WAVE + lane;

@@ -52,7 +52,7 @@ __global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__
 }
 
 template <bool INCL>
-__global__ __launch_bounds__(NT) void k_tile_pass_seg(const int* __restrict__ in,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_tile_pass_seg(const int* __restrict__ in,
                                                       int* __restrict__ out, int n,
                                                       TileOut to, Levels L, SegIn sg) {
     __shared__ TileSmem sm;
