@@ -183,6 +183,33 @@ def test_adversarial_deep_and_overflow(torch_mod, algo):
 
 
 @pytest.mark.parametrize("algo", ALGOS)
+def test_query_queue_overflow(torch_mod, algo):
+    """More in-tile queries than a wave's queue holds (192): lanes whose 16
+    values descend, so that every element but the first asks the previous
+    lane.  Such waves store their outputs lane by lane; the others stage them
+    through LDS and store them coalesced.  Both kinds in one tile, every
+    wave of the tile, and a partial last tile."""
+    import apsep_amd as A
+    torch = torch_mod
+    n = (1 << 21) + 777
+    i = np.arange(n, dtype=np.int64)
+    rng = np.random.default_rng(11)
+    desc16 = (1_000_000 - (i % 16)).astype(np.int32)
+    rnd = rng.integers(0, 2**31 - 1, size=n, dtype=np.int64).astype(np.int32)
+    wave = (i // 1024) % 2 == 0  # 1024 elements per wave of 64 lanes
+    cases = {
+        "all-waves-overflow": desc16,
+        "alternate-waves": np.where(wave, desc16, rnd).astype(np.int32),
+        "overflow-small-values": np.where(wave, desc16 - 2_000_000, rnd).astype(np.int32),
+    }
+    with A.Scratch(algo, n) as s:
+        for name, a in cases.items():
+            for incl in (False, True):
+                _log(algo, name, incl)
+                check(solve_on(s, torch, a, incl), a, incl, "%s %s" % (algo, name))
+
+
+@pytest.mark.parametrize("algo", ALGOS)
 @pytest.mark.parametrize("n,kinds", [
     (1 << 20, ("random", "descending", "ascending")),
     (1 << 25, ("random", "descending", "ascending")),
