@@ -686,6 +686,34 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     }
     __syncthreads();  // no wave reads the tile's values past this point
 
+    // 4. open elements: (position, value) pairs in position order, up to QS
+    int obase = ooff;
+#pragma unroll
+    for (int w = 0; w < NWAVE - 1; w++)
+        if (w < wave) obase += sm.wopen[w];
+    if (omask && obase < QS) {
+#pragma unroll
+        for (int i = 0; i < K; i++)
+            if (omask & (1u << i)) {
+                if (obase < QS)
+                    to.qlist[(size_t)t * QS + obase] = make_int2(tid * K + i, v[i]);
+                obase++;
+            }
+    }
+    // 5. right spine: ranks from the tile's end, the first RS recorded
+    int srank = safter_w;
+#pragma unroll
+    for (int w = 1; w < NWAVE; w++)
+        if (w > wave) srank += sm.wspine[w];
+    if (smask && srank < RS) {
+#pragma unroll
+        for (int i = K - 1; i >= 0; i--)
+            if (smask & (1u << i)) {
+                if (srank < RS)
+                    to.spine[(size_t)t * RS + srank] = make_int2(tid * K + i, v[i]);
+                srank++;
+            }
+    }
     // staged outputs.  Each wave stages its 1024 outputs in its own lanes'
     // value slots: 16-byte group g of the wave (elements 4g .. 4g+3) goes to
     // val4[g & 3][wave * 64 + ((g >> 2) ^ ((g & 3) << 2))].  The owner writes
@@ -729,34 +757,6 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         }
     }
 
-    // 4. open elements: (position, value) pairs in position order, up to QS
-    int obase = ooff;
-#pragma unroll
-    for (int w = 0; w < NWAVE - 1; w++)
-        if (w < wave) obase += sm.wopen[w];
-    if (omask && obase < QS) {
-#pragma unroll
-        for (int i = 0; i < K; i++)
-            if (omask & (1u << i)) {
-                if (obase < QS)
-                    to.qlist[(size_t)t * QS + obase] = make_int2(tid * K + i, v[i]);
-                obase++;
-            }
-    }
-    // 5. right spine: ranks from the tile's end, the first RS recorded
-    int srank = safter_w;
-#pragma unroll
-    for (int w = 1; w < NWAVE; w++)
-        if (w > wave) srank += sm.wspine[w];
-    if (smask && srank < RS) {
-#pragma unroll
-        for (int i = K - 1; i >= 0; i--)
-            if (smask & (1u << i)) {
-                if (srank < RS)
-                    to.spine[(size_t)t * RS + srank] = make_int2(tid * K + i, v[i]);
-                srank++;
-            }
-    }
     if (tid == 0) {
         int m = sm.wmin[0], total = sm.wopen[0], stotal = sm.wspine[0];
 #pragma unroll
