@@ -583,8 +583,8 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         if (lmin < A || tid == NT - 1) {
             int m = A;
 #pragma unroll
-            for (int i = K - 1; i >= 0; i--) {
-                smask |= (v[i] < m ? 1u : 0u) << i;
+            for (int i = K - 1; i >= 0; i--) {  // bit i = v[i] < m (sign bit, one op)
+                smask = __builtin_amdgcn_alignbit(smask, (unsigned)__builtin_elementwise_sub_sat(v[i], m), 31);
                 m = min(m, v[i]);
             }
             if (tid == NT - 1) smask |= 1u << (K - 1);
