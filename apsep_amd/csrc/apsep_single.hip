@@ -23,17 +23,13 @@ namespace apsep {
 // Kernels
 // ---------------------------------------------------------------------------
 
-// WSTL pass 1: one workgroup per tile (also accumulates level 1).  Workgroup
-// b solves tile b * stride mod nt (stride coprime to nt): workgroups that run
-// at the same time then belong to different 64-tile groups, and the level-1
-// atomics of concurrent tiles hit different keys instead of queueing on one.
+// WSTL pass 1: one workgroup per tile (also accumulates level 1).
 template <bool INCL>
 __global__ __launch_bounds__(NT) void k_tile_pass(const int* __restrict__ in,
                                                   int* __restrict__ out, int n,
-                                                  TileOut to, Levels L, int stride) {
+                                                  TileOut to, Levels L) {
     __shared__ TileSmem sm;
-    const int t = (int)((long long)blockIdx.x * stride % L.nt);
-    tile_step<INCL>(in, out, n, t, sm, to, L);
+    tile_step<INCL>(in, out, n, (int)blockIdx.x, sm, to, L);
 }
 
 // WSTL pass 2 (only when level 1 exceeds UPPER_MIN entries): levels >= 2.
@@ -88,21 +84,6 @@ void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
 // ---------------------------------------------------------------------------
 
 static inline int64_t round4(int64_t x) { return (x + 3) & ~(int64_t)3; }
-
-// Tile order of the tile pass: the smallest stride >= FAN + 1 coprime to nt
-// (1 when a single group holds every tile).
-static int tile_stride(int nt) {
-    if (nt <= FAN) return 1;
-    for (int s = FAN + 1;; s++) {
-        int a = s, b = nt;
-        while (b) {
-            const int r = a % b;
-            a = b;
-            b = r;
-        }
-        if (a == 1) return s;
-    }
-}
 
 // Level sizes for nt tiles; returns nlev.
 int level_sizes(int nt, int* sizes) {
@@ -239,8 +220,7 @@ static apsep_status run_wstl(apsep_scratch* s, const int* in, int* out, int n,
     Levels L;
     if (fill_levels(s, nt, L, st) != APSEP_OK) return APSEP_ERR_LAUNCH;
     TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
-    hipLaunchKernelGGL(k_tile_pass<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L,
-                           tile_stride(nt));
+    hipLaunchKernelGGL(k_tile_pass<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L);
     if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
     if (nt > 1 && L.upper)
         hipLaunchKernelGGL(k_upper_levels, dim3(1), dim3(256), 0, st, L);
@@ -285,8 +265,7 @@ static apsep_status time_wstl(apsep_scratch* s, const int* in, int* out, int n, 
     // level-1 atomicMin sees the same keys, the fix-up the same summaries
     (void)hipEventRecord(ev[0], st);
     for (int r = 0; r < reps; r++)
-        hipLaunchKernelGGL(k_tile_pass<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L,
-                           tile_stride(nt));
+        hipLaunchKernelGGL(k_tile_pass<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L);
     (void)hipEventRecord(ev[1], st);
     if (nt > 1 && L.upper)
         for (int r = 0; r < reps; r++)
