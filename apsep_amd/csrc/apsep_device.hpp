@@ -479,7 +479,7 @@ __device__ __forceinline__ unsigned qual_bits(int p, const int* v) {
     return INCL ? (~m & 0xFFFFu) : m;
 }
 
-template <bool INCL, bool SEG = false>
+template <bool INCL, bool SEG = false, bool L1 = true>
 __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, const int* v,
                                            TileSmem& sm, const TileOut& to, const Levels& L,
                                            const SegLane sl = SegLane{0, 0u}) {
@@ -768,19 +768,21 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         L.mn0[t] = m;
         to.qcount[t] = total;
         to.scount[t] = stotal;
-        if (L.nlev > 1) atomicMin(&L.key[1][t >> LOG_FAN], make_key(L.epoch, m));
+        if (L1 && L.nlev > 1) atomicMin(&L.key[1][t >> LOG_FAN], make_key(L.epoch, m));
     }
     __syncthreads();  // sm is reused by the next tile
 }
 
-// Solve tile t (loads included) in the calling workgroup.
-template <bool INCL>
+// Solve tile t (loads included) in the calling workgroup.  With L1 the tile
+// also folds its min into level 1 (an atomic); without, build_level1 makes
+// level 1 from the tile mins after the pass.
+template <bool INCL, bool L1 = true>
 __device__ __forceinline__ void tile_step(const int* __restrict__ in, int* __restrict__ out,
                                           int n, int t, TileSmem& sm, const TileOut& to,
                                           const Levels& L) {
     int v[K];
     load_lane(in, n, t, v);
-    solve_tile<INCL>(out, n, t, v, sm, to, L);
+    solve_tile<INCL, false, L1>(out, n, t, v, sm, to, L);
 }
 
 // Segmented tile t: mark the segment starts inside the tile in LDS, give
@@ -830,13 +832,25 @@ __device__ __forceinline__ void tile_step_seg(const int* __restrict__ in, int* _
 #pragma unroll
     for (int w = 0; w < NWAVE - 1; w++)
         if (w < wave) incoming = max(incoming, sm.wlast[w]);
-    solve_tile<INCL, true>(out, n, t, v, sm, to, L, SegLane{incoming, wm});
+    solve_tile<INCL, true, false>(out, n, t, v, sm, to, L, SegLane{incoming, wm});
 }
 
 // ---------------------------------------------------------------------------
 // Levels >= 2 (only when level 1 exceeds UPPER_MIN entries): one workgroup
 // builds them from the level below after the tile pass.
 // ---------------------------------------------------------------------------
+// Level 1 from the tile mins, one wave per 64-tile group (after a tile pass
+// without level-1 atomics).  Atomics from the tile pass itself queue up:
+// the 64 tiles of a group run at the same time and update one key.
+__device__ __forceinline__ void build_level1(const Levels& L) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int g = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (g >= L.n[1]) return;  // uniform per wave
+    const int idx = g * FAN + lane;
+    const int m = wave_min(idx < L.nt ? L.mn0[idx] : INF);
+    if (lane == 0) L.key[1][g] = make_key(L.epoch, m);
+}
+
 __device__ __forceinline__ void build_upper_levels(const Levels& L) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;

@@ -5,10 +5,12 @@
 // The reference has no batched entry point; its callers loop over arrays
 // with one launchWSTL / launchSPT each (src/apsep.cuh:2598-2605,
 // 3069-3076).  Here the segments are concatenated and solved as one array:
-//   setup     tile -> segment map, level-1 keys reset
+//   setup     tile -> segment map and bounds, level-1 keys reset
 //   tile pass the single-array tile solver with the segment starts inside
 //             each tile marked in LDS; answers rebased to their segments
-//   [levels]  as for one array
+//   levels    level 1 from the tile mins (no atomics in the tile pass: a
+//             batch's 64-tile groups would queue on their keys), [levels
+//             >= 2 as for one array]
 //   fix-up    prefix-min and searches bounded by the segment that runs into
 //             the tile from before
 // The level-1 keys are reset by the setup kernel (a kernel, not a memset,
@@ -59,6 +61,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     tile_step_seg<INCL>(in, out, n, (int)blockIdx.x, sm, to, L, sg);
 }
 
+__global__ __launch_bounds__(256) void k_level1_seg(Levels L) { build_level1(L); }
 __global__ __launch_bounds__(256) void k_upper_levels_seg(Levels L) { build_upper_levels(L); }
 
 template <bool INCL>
@@ -98,6 +101,7 @@ static apsep_status seg_launch(apsep_batch_plan* p, const int* in, const long lo
                        p->tbound, nt, nkey1 ? s->key[1] : nullptr, nkey1);
     hipLaunchKernelGGL(k_tile_pass_seg<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L, sg);
     if (nt > 1) {
+        if (L.nlev > 1) hipLaunchKernelGGL(k_level1_seg, dim3((L.n[1] + 3) / 4), dim3(256), 0, st, L);
         if (L.upper) hipLaunchKernelGGL(k_upper_levels_seg, dim3(1), dim3(256), 0, st, L);
         constexpr int tpw = SEG_FIX_WG / WAVE;
         hipLaunchKernelGGL(k_fix_pass_seg<INCL>, dim3((nt + tpw - 1) / tpw), dim3(SEG_FIX_WG), 0,

@@ -23,7 +23,8 @@ namespace apsep {
 // Kernels
 // ---------------------------------------------------------------------------
 
-// WSTL pass 1: one workgroup per tile (also accumulates level 1).
+// WSTL pass 1: one workgroup per tile (also accumulates level 1: a separate
+// level-1 kernel, as the segmented pass uses, measured slower here).
 template <bool INCL>
 __global__ __launch_bounds__(NT) void k_tile_pass(const int* __restrict__ in,
                                                   int* __restrict__ out, int n,
