@@ -12,10 +12,12 @@
 //            inter-tile search.
 //   levels = 64-ary min hierarchy over tile mins.  Level 0 (tile mins) is a
 //            plain array; levels >= 1 hold 64-bit keys (run epoch in the high
-//            word, biased min in the low word).  The tile pass accumulates
-//            level 1 itself with device-scope atomicMin (64 tiles per key),
-//            with no reset between runs and no memory fence: a key from an
-//            older run compares larger and decodes as empty.  Levels >= 2 are
+//            word, biased min in the low word).  The tile pass of a plain
+//            solve accumulates level 1 itself with device-scope atomicMin
+//            (64 tiles per key), with no reset between runs and no memory
+//            fence: a key from an older run compares larger and decodes as
+//            empty.  Segmented batches build level 1 from the tile mins in
+//            a kernel of its own (build_level1).  Levels >= 2 are
 //            built by a one-workgroup pass only when level 1 exceeds
 //            UPPER_MIN entries; below that, readers fold level 1 directly.
 //
@@ -33,7 +35,10 @@
 //   4. the rest are the tile's "open" elements: the tile's prefix minima.
 //      They are written as -1 and recorded (up to QS (position, value) pairs
 //      + count) for the fix-up phase, which only revisits the ones whose
-//      value beats the exclusive prefix-min of all earlier tiles.
+//      value beats the exclusive prefix-min of all earlier tiles;
+//   5. the right spine (strict suffix minima, up to RS by rank) is recorded
+//      for the fix-up's searches into this tile;
+//   6. each wave stages its answers in LDS and stores them coalesced.
 //
 // Invariants relied on (reference reasoning, docs/spt_design.md:140-151 and
 // docs/spt_early_out_regression.md:42-51): open elements of a tile are its
