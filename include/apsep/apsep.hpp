@@ -13,7 +13,7 @@
 //   * T must be int (the reference's SPT is int-only in practice: its int4
 //     loads value-convert lanes, src/apsep.cuh:2689-2691).
 //   * BLOCK_SIZE and IPT are accepted and ignored: the gfx950 engine uses its
-//     own wave64 tile shape (256 lanes x 16 elements).
+//     own wave64 tile shape (512 lanes x 16 elements).
 //   * run* launch on the default stream, asynchronously, like the reference;
 //     an optional trailing stream argument (hipStream_t as void*) is added.
 //   * Errors print "APSEP error <file>:<line>: <text>" and exit, as the
@@ -42,7 +42,7 @@ inline void check(apsep_status st, const char* file, int line) {
 // Scratch handle plus the reference's informational fields.
 struct ScratchBase {
     apsep_scratch* handle = nullptr;
-    int num_blocks = 0;   // tiles (the engine's tile = 4096 elements)
+    int num_blocks = 0;   // tiles (the engine's tile = 8192 elements)
     int num_phys = 0;     // SPT persistent workgroups (occupancy x CUs)
     int n_cap = 0;
 };
