@@ -209,16 +209,6 @@ __device__ __forceinline__ int wave_incl_min(int x) {
     return x;
 }
 
-__device__ __forceinline__ int wave_incl_add(int x) {
-    const int lane = threadIdx.x & (WAVE - 1);
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        const int y = wave_shfl_up(x, d);
-        if (lane >= d) x += y;
-    }
-    return x;
-}
-
 // Min over the wave (DPP row scans, then the row carries; no LDS round
 // trip).  The whole wave must be active.
 __device__ __forceinline__ int wave_min(int x) {
@@ -275,24 +265,6 @@ __device__ __forceinline__ int last_in_32(const int* __restrict__ arr, int b, in
     int r = -1;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        r = qual<INCL>(w[k].x, v) ? b + 4 * k + 0 : r;
-        r = qual<INCL>(w[k].y, v) ? b + 4 * k + 1 : r;
-        r = qual<INCL>(w[k].z, v) ? b + 4 * k + 2 : r;
-        r = qual<INCL>(w[k].w, v) ? b + 4 * k + 3 : r;
-    }
-    return r;
-}
-
-// Rightmost qualifying index in the aligned 64-entry block [b, b+64), all
-// 16 loads issued together; -1 if none.
-template <bool INCL>
-__device__ __forceinline__ int last_in_64(const int* __restrict__ arr, int b, int v) {
-    int4 w[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) w[k] = *reinterpret_cast<const int4*>(arr + b + 4 * k);
-    int r = -1;
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
         r = qual<INCL>(w[k].x, v) ? b + 4 * k + 0 : r;
         r = qual<INCL>(w[k].y, v) ? b + 4 * k + 1 : r;
         r = qual<INCL>(w[k].z, v) ? b + 4 * k + 2 : r;
