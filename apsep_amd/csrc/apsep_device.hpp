@@ -379,7 +379,7 @@ __device__ __forceinline__ int intile_answer(const TileSmem& sm, int qt, int val
 // This lane's K values of tile t (INF past the end of the array).
 __device__ __forceinline__ void load_lane(const int* __restrict__ in, int n, int t, int* v) {
     const int base = t * TILE + (int)threadIdx.x * K;
-    if (t * TILE + TILE <= n) {
+    if (t * TILE <= n - TILE) {  // (not t * TILE + TILE: it overflows near 2^31)
 #pragma unroll
         for (int q = 0; q < K4; q++) {
             const int4 a = *reinterpret_cast<const int4*>(in + base + 4 * q);
@@ -773,7 +773,7 @@ __device__ __forceinline__ void tile_step_seg(const int* __restrict__ in, int* _
     const int lane = tid & (WAVE - 1);
     const int wave = tid >> 6;
     const int tile_start = t * TILE;
-    const int tile_end = min(tile_start + TILE, n);
+    const int tile_end = tile_start + min(TILE, n - tile_start);  // no overflow near 2^31
     int v[K];
     load_lane(in, n, t, v);
     // offsets are clamped so that malformed ones cannot send a read astray
@@ -1194,10 +1194,12 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     // the tile qualifies for it (running prefix-min rm).
     int rm = INF;
     bool have = false;  // rm holds a real element (INCL: INF is a legal key)
-    const int tile_end = min(tile_start + lim, n);
-    for (int p0 = tile_start; p0 < tile_end; p0 += WAVE) {
-        const int idx = p0 + lane;
-        const bool valid = idx < tile_end;
+    // (offsets from the tile start: p0 + 64 would overflow near 2^31)
+    const int span = min(lim, n - tile_start);
+    for (int o = 0; o < span; o += WAVE) {
+        const int p0 = tile_start + o;
+        const bool valid = lane < span - o;
+        const int idx = valid ? p0 + lane : p0;
         const int xv = valid ? in[idx] : INF;
         const int inc = wave_incl_min(xv);
         int ex = wave_shfl_up(inc, 1);

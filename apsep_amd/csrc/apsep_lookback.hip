@@ -251,10 +251,12 @@ __global__ __launch_bounds__(NT) void k_lookback(const int* __restrict__ in,
     // more open elements than recorded: rescan the tile (prefix minima)
     int rm = INF;
     bool have = false;
-    const int tile_end = min(tile_start + TILE, n);
-    for (int p0 = tile_start; p0 < tile_end; p0 += WAVE) {
-        const int idx = p0 + lane;
-        const bool valid = idx < tile_end;
+    // (offsets from the tile start: p0 + 64 would overflow near 2^31)
+    const int span = min(TILE, n - tile_start);
+    for (int o = 0; o < span; o += WAVE) {
+        const int p0 = tile_start + o;
+        const bool valid = lane < span - o;
+        const int idx = valid ? p0 + lane : p0;
         const int xv = valid ? in[idx] : INF;
         const int inc = wave_incl_min(xv);
         int ex = wave_shfl_up(inc, 1);
@@ -288,7 +290,7 @@ __global__ void k_lookback_begin(LbCtl* c) {
 template <bool INCL>
 static apsep_status run_lookback_t(apsep_scratch* s, const int* in, int* out, int n,
                                    hipStream_t st) {
-    const int nt = (n + TILE - 1) / TILE;
+    const int nt = (int)(((long long)n + TILE - 1) / TILE);  // (n + TILE - 1 overflows near 2^31)
     Levels L{};
     L.nt = nt;
     L.nlev = 1;  // no level-1 atomics from the tile solver

@@ -78,7 +78,7 @@ template <bool INCL>
 static apsep_status seg_launch(apsep_batch_plan* p, const int* in, const long long* offs,
                                int* out, int nseg, int n, hipStream_t st) {
     apsep_scratch* s = p->s;
-    const int nt = (n + TILE - 1) / TILE;
+    const int nt = (int)(((long long)n + TILE - 1) / TILE);  // (n + TILE - 1 overflows near 2^31)
     Levels L;
     int sz[MAX_LEVELS];
     L.nt = nt;

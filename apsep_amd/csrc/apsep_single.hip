@@ -217,7 +217,7 @@ static apsep_status fill_levels(apsep_scratch* s, int nt, Levels& L, hipStream_t
 template <bool INCL>
 static apsep_status run_wstl(apsep_scratch* s, const int* in, int* out, int n,
                              hipStream_t st) {
-    const int nt = (n + TILE - 1) / TILE;
+    const int nt = (int)(((long long)n + TILE - 1) / TILE);  // (n + TILE - 1 overflows near 2^31)
     Levels L;
     if (fill_levels(s, nt, L, st) != APSEP_OK) return APSEP_ERR_LAUNCH;
     TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
@@ -237,7 +237,7 @@ static apsep_status run_wstl(apsep_scratch* s, const int* in, int* out, int n,
 template <bool INCL>
 static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
                             hipStream_t st) {
-    const int nt = (n + TILE - 1) / TILE;
+    const int nt = (int)(((long long)n + TILE - 1) / TILE);  // (n + TILE - 1 overflows near 2^31)
     Levels L;
     if (fill_levels(s, nt, L, st) != APSEP_OK) return APSEP_ERR_LAUNCH;
     TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
@@ -257,7 +257,7 @@ static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
 template <bool INCL>
 static apsep_status time_wstl(apsep_scratch* s, const int* in, int* out, int n, int reps,
                               hipStream_t st, hipEvent_t* ev) {
-    const int nt = (n + TILE - 1) / TILE;
+    const int nt = (int)(((long long)n + TILE - 1) / TILE);  // (n + TILE - 1 overflows near 2^31)
     Levels L;
     if (fill_levels(s, nt, L, st) != APSEP_OK) return APSEP_ERR_LAUNCH;
     TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};

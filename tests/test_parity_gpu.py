@@ -183,6 +183,47 @@ def test_adversarial_deep_and_overflow(torch_mod, algo):
 
 
 @pytest.mark.parametrize("algo", ALGOS)
+def test_largest_size(torch_mod, algo):
+    """n = 2^31 - 1, the largest size the ABI accepts: the last tile is
+    partial and ends at the int32 limit.  Closed forms checked on the
+    device: ascending (answer i - 1), then the last tile replaced by a
+    descent from 10^9 (all its elements open and beaten by the prefix min 0,
+    so the fix-up rescans that tile; the answer of value v is index v - 1,
+    or v under "<=")."""
+    import apsep_amd as A
+    torch = torch_mod
+    n = (1 << 31) - 1
+    x = torch.arange(n, dtype=torch.int32, device="cuda")
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    t0 = (n // 8192) * 8192  # start of the last (partial) tile
+    with A.Scratch(algo, n) as s:
+        for incl in (False, True):
+            _log(algo, "largest ascending", incl)
+            out.fill_(int(POISON))
+            s.run(x, out, incl=incl)
+            torch.cuda.synchronize()
+            s.check()
+            assert out[0].item() == -1
+            assert bool(torch.equal(out[1:], x[:-1])), "ascending n=2^31-1 incl=%d" % incl
+        tail = 1_000_000_000 - torch.arange(n - t0, dtype=torch.int32, device="cuda")
+        x[t0:] = tail
+        for incl in (False, True):
+            _log(algo, "largest descending last tile", incl)
+            out.fill_(int(POISON))
+            s.run(x, out, incl=incl)
+            torch.cuda.synchronize()
+            s.check()
+            assert out[0].item() == -1
+            head = torch.arange(1, t0, dtype=torch.int32, device="cuda")
+            assert bool(torch.equal(out[1:t0], head - 1)), "head n=2^31-1 incl=%d" % incl
+            assert bool(torch.equal(out[t0:], tail - (0 if incl else 1))), \
+                "last tile n=2^31-1 incl=%d" % incl
+            del head
+    del x, out
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("algo", ALGOS)
 def test_query_queue_overflow(torch_mod, algo):
     """More in-tile queries than a wave's queue holds (192): lanes whose 16
     values descend, so that every element but the first asks the previous
