@@ -29,9 +29,9 @@
 //      6-step binary lifting per query;
 //   3. an element unanswered in its lane is answered inside the tile iff the
 //      exclusive prefix-min of the earlier lanes qualifies: the lane's first
-//      element is checked against the previous lane's 16 values, every other
-//      such element goes through a per-wave LDS queue whose entries all 64
-//      lanes resolve by lifting, one query per lane per round;
+//      XQ + 1 such elements are checked against the previous lane's 16
+//      values, the others go through a per-wave LDS queue whose entries all
+//      64 lanes resolve by lifting, one query per lane per round;
 //   4. the rest are the tile's "open" elements: the tile's prefix minima.
 //      They are written as -1 and recorded (up to QS (position, value) pairs
 //      + count) for the fix-up phase, which only revisits the ones whose
@@ -67,6 +67,7 @@ constexpr int CPT = TILE / CHUNK;  // 128 chunk mins per tile
 constexpr int QS = 32;             // recorded open elements per tile
 constexpr int RS = 32;             // recorded right-spine entries per tile
 constexpr int QCAP = 192;          // per-wave LDS queue of cross-lane queries
+constexpr int XQ = 3;              // further queries per lane tried on the previous lane
 constexpr int FAN = 64;            // hierarchy fan-out
 constexpr int LOG_FAN = 6;
 constexpr int MAX_LEVELS = 5;      // 64^5 tiles >> 2^31 / TILE
@@ -571,11 +572,38 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
 
     // 3a. the lane's first element (always unanswered in-lane) against the
     //     previous lane's values
+    bool near = false;  // its answer is in the previous lane
     if (qmask & 1u) {
         const int r = last_in_lane<INCL>(sm, tid - 1, v[0]);
         if (r >= 0) {
             res[0] = base - K + r;
             qmask &= ~1u;
+            near = true;
+        }
+    }
+    // Then up to XQ more of the lane's queries, in position order.  They are
+    // prefix minima of the lane, so each is at most as large as the one
+    // before and its answer lies no later: once one has no answer in the
+    // previous lane, no later one has.  (When the first element is not a
+    // query, no element of the lane is.)  On uniform random keys this
+    // settles about two thirds of the queue's entries, at a third of a queue
+    // round's cost each.  The low 16 bits of xa mark the elements so
+    // answered; the answer of the k-th of them is at offset
+    // (xa >> (16 + 4k)) & 15 of the previous lane.  They are written with
+    // the staged or patched outputs below.  (Not in segmented solves: their
+    // tile pass has no registers to spare for it.)
+    unsigned xa = 0;
+#pragma unroll
+    for (int k = 0; k < (SEG ? 0 : XQ); k++) {
+        if (near && qmask) {
+            const int i = __builtin_ctz(qmask);
+            const int r = last_in_lane<INCL>(sm, tid - 1, tile_value(sm, tid * K + i));
+            if (r >= 0) {
+                xa |= (1u << i) | ((unsigned)r << (16 + 4 * k));
+                qmask &= qmask - 1;
+            } else {
+                near = false;
+            }
         }
     }
 
@@ -626,6 +654,15 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
             for (int i = 0; i < K; i++)
                 if (i < nvalid) out[base + i] = seg_rel<SEG>(res[i], i, base, sl);
         }
+        unsigned xm = xa & 0xFFFFu;
+#pragma unroll
+        for (int k = 0; k < (SEG ? 0 : XQ); k++)
+            if (xm) {
+                const int i = __builtin_ctz(xm);
+                xm &= xm - 1;
+                out[base + i] = seg_rel<SEG>(base - K + (int)((xa >> (16 + 4 * k)) & 15u), i,
+                                             base, sl);
+            }
     }
 
     // 3b. the rest through the wave's queue: compact, resolve one query per
@@ -717,6 +754,15 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
                 seg_rel<SEG>(tile_start + (int)sm.q[wave][k], i, base, sl);
             k++;
         }
+        unsigned xm = xa & 0xFFFFu;
+#pragma unroll
+        for (int x = 0; x < (SEG ? 0 : XQ); x++)
+            if (xm) {
+                const int i = __builtin_ctz(xm), q = i >> 2;
+                xm &= xm - 1;
+                sv[(q * NT + wave * WAVE + (lane ^ (q << 2))) * 4 + (i & 3)] =
+                    seg_rel<SEG>(base - K + (int)((xa >> (16 + 4 * x)) & 15u), i, base, sl);
+            }
         wave_lds_sync();
         const int wbase = tile_start + wave * (WAVE * K);
 #pragma unroll

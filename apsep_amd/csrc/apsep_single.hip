@@ -26,7 +26,7 @@ namespace apsep {
 // WSTL pass 1: one workgroup per tile (also accumulates level 1: a separate
 // level-1 kernel, as the segmented pass uses, measured slower here).
 template <bool INCL>
-__global__ __launch_bounds__(NT) void k_tile_pass(const int* __restrict__ in,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_tile_pass(const int* __restrict__ in,
                                                   int* __restrict__ out, int n,
                                                   TileOut to, Levels L) {
     __shared__ TileSmem sm;
