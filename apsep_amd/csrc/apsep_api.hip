@@ -50,12 +50,15 @@ apsep_status apsep_scratch_bytes(apsep_algo algo, int64_t n_cap, int incl,
     if (!bytes || !valid_algo(algo) || n_cap < 0 || (incl != 0 && incl != 1))
         return APSEP_ERR_INVALID_ARG;
     if (n_cap > INT32_MAX) return APSEP_ERR_TOO_LARGE;
-    *bytes = n_cap > 0 ? single_scratch_bytes(n_cap, resolve_algo(algo)) : 0;
+    *bytes = n_cap > 0 ? single_scratch_bytes(n_cap, resolve_algo(algo), TILE) : 0;
     return APSEP_OK;
 }
 
-apsep_status apsep_scratch_create(apsep_algo algo, int64_t n_cap, int incl,
-                                  apsep_scratch** out) {
+// A scratch whose per-tile arrays are sized for tiles of `tile` elements:
+// the single-array solvers' tile for apsep_scratch_create, the segmented
+// solver's for a batch plan.
+static apsep_status scratch_create(apsep_algo algo, int64_t n_cap, int incl, int tile,
+                                   apsep_scratch** out) {
     if (!out) return APSEP_ERR_INVALID_ARG;
     *out = nullptr;
     if (!valid_algo(algo) || n_cap < 0 || (incl != 0 && incl != 1))
@@ -73,6 +76,7 @@ apsep_status apsep_scratch_create(apsep_algo algo, int64_t n_cap, int incl,
     s->algo = resolve_algo(algo);
     s->incl = incl;
     s->n_cap = n_cap;
+    s->tile = tile;
     s->device = dev;
     if (hipDeviceGetAttribute(&s->num_cus, hipDeviceAttributeMultiprocessorCount, dev) !=
         hipSuccess) {
@@ -87,6 +91,11 @@ apsep_status apsep_scratch_create(apsep_algo algo, int64_t n_cap, int incl,
     }
     *out = s;
     return APSEP_OK;
+}
+
+apsep_status apsep_scratch_create(apsep_algo algo, int64_t n_cap, int incl,
+                                  apsep_scratch** out) {
+    return scratch_create(algo, n_cap, incl, TILE, out);
 }
 
 apsep_status apsep_scratch_destroy(apsep_scratch* s) {
@@ -105,7 +114,7 @@ apsep_status apsep_scratch_query(const apsep_scratch* s, apsep_scratch_info* inf
     info->algo = (int32_t)s->algo;
     info->incl = s->incl;
     info->device = s->device;
-    info->tile_elems = TILE;
+    info->tile_elems = s->tile;
     info->tiles = s->nt_cap;
     info->levels = s->nt_cap > 0 ? level_sizes(s->nt_cap, sz) : 0;
     info->persistent_blocks = s->coop_grid;
@@ -188,12 +197,13 @@ apsep_status apsep_batch_plan_create(int64_t total_cap, int32_t nseg_cap, int in
     if (!p) return APSEP_ERR_ALLOC;
     p->total_cap = total_cap;
     p->nseg_cap = nseg_cap;
-    apsep_status st = apsep_scratch_create(APSEP_ALGO_WSTL, total_cap, incl, &p->s);
+    const int tile = seg_tile_elems();
+    apsep_status st = scratch_create(APSEP_ALGO_WSTL, total_cap, incl, tile, &p->s);
     if (st != APSEP_OK) {
         delete p;
         return st;
     }
-    const int64_t tiles = (total_cap + TILE - 1) / TILE;
+    const int64_t tiles = (total_cap + tile - 1) / tile;
     if (tiles > 0 && (hipMalloc((void**)&p->tmap, (size_t)tiles * sizeof(int)) != hipSuccess ||
                       hipMalloc((void**)&p->tbound, (size_t)tiles * sizeof(int2)) != hipSuccess)) {
         (void)hipGetLastError();

@@ -54,8 +54,11 @@
 
 namespace apsep {
 
+#ifndef APSEP_TILE_LANES
+#define APSEP_TILE_LANES 512
+#endif
 constexpr int WAVE = 64;
-constexpr int NT = 512;            // lanes per tile workgroup
+constexpr int NT = APSEP_TILE_LANES;  // lanes per tile workgroup (256 or 512)
 constexpr int K = 16;              // consecutive elements per lane
 constexpr int K4 = K / 4;          // dwordx4 per lane
 constexpr int TILE = NT * K;       // 8192 elements per tile
@@ -76,6 +79,7 @@ constexpr int INF = INT_MAX;
 #define SPT_WAVES_PER_EU 4         // occupancy the persistent kernel is pinned at
 
 static_assert(K == 16 && LPC == 4 && (1 << LOG_LPC) == LPC, "tile shape");
+static_assert(NWAVE == 4 || NWAVE == 8, "the wave searches handle 4 or 8 waves per tile");
 static_assert(CPT % 16 == 0, "chunk-min search works in blocks of 16");
 
 typedef unsigned long long u64;
@@ -198,6 +202,11 @@ constexpr int DPP_ROW_BCAST31 = 0x143;  // lane 31, read by rows 2 and 3
 template <int CTRL, int ROWS = 0xf>
 __device__ __forceinline__ int dpp(int ident, int x) {
     return __builtin_amdgcn_update_dpp(ident, x, CTRL, ROWS, 0xf, false);
+}
+// Row shift with lanes whose source lies outside the row reading 0 (sums).
+template <int CTRL>
+__device__ __forceinline__ int dpp0(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, true);
 }
 __device__ __forceinline__ int readlane(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
 __device__ __forceinline__ int lane63(int x) { return __builtin_amdgcn_readlane(x, WAVE - 1); }
@@ -338,7 +347,6 @@ __device__ __forceinline__ int intile_answer(const TileSmem& sm, int qt, int val
     const int a9 = sm.win[3][max(qt - 9, 0)];
     const int4 rq = sm.rmin[wq];
     const int4 wa = *reinterpret_cast<const int4*>(&sm.wmin[0]);
-    const int4 wb = *reinterpret_cast<const int4*>(&sm.wmin[4]);
     // min of the own row's lanes before qt: two 8-lane windows
     const int rowpre = min(a1, rl > 8 ? a9 : INF);
     // nearest earlier row of the own wave holding a qualifying element
@@ -352,10 +360,13 @@ __device__ __forceinline__ int intile_answer(const TileSmem& sm, int qt, int val
     w = ((wq > 1) & qual<INCL>(wa.y, val)) ? 1 : w;
     w = ((wq > 2) & qual<INCL>(wa.z, val)) ? 2 : w;
     w = ((wq > 3) & qual<INCL>(wa.w, val)) ? 3 : w;
-    w = ((wq > 4) & qual<INCL>(wb.x, val)) ? 4 : w;
-    w = ((wq > 5) & qual<INCL>(wb.y, val)) ? 5 : w;
-    w = ((wq > 6) & qual<INCL>(wb.z, val)) ? 6 : w;
-    w = ((wq > 7) & qual<INCL>(wb.w, val)) ? 7 : w;
+    if constexpr (NWAVE > 4) {
+        const int4 wb = *reinterpret_cast<const int4*>(&sm.wmin[4]);
+        w = ((wq > 4) & qual<INCL>(wb.x, val)) ? 4 : w;
+        w = ((wq > 5) & qual<INCL>(wb.y, val)) ? 5 : w;
+        w = ((wq > 6) & qual<INCL>(wb.z, val)) ? 6 : w;
+        w = ((wq > 7) & qual<INCL>(wb.w, val)) ? 7 : w;
+    }
     const int4 rw = sm.rmin[w];
     const int rw_last = qual<INCL>(rw.w, val) ? 3 : qual<INCL>(rw.z, val) ? 2
                       : qual<INCL>(rw.y, val) ? 1 : 0;
@@ -428,6 +439,41 @@ __device__ __forceinline__ void sel4(int& r, int a0, int a1, int a2, int a3, int
               [j0] "I"(J0), [j1] "I"(J0 + 1), [j2] "I"(J0 + 2), [j3] "I"(J0 + 3));
 }
 
+// sel4 starting from `seed` instead of r: the first select reads the seed
+// and writes r, so a seed shared by many elements is not copied per element.
+template <bool INCL, int J0>
+__device__ __forceinline__ int sel4_from(int seed, int a0, int a1, int a2, int a3, int x) {
+    int r;
+    u64 m0, m1, m2, m3;
+    if (INCL)
+        asm volatile(
+            "v_cmp_le_i32_e64 %[m0], %[a0], %[x]\n"
+            "v_cmp_le_i32_e64 %[m1], %[a1], %[x]\n"
+            "v_cmp_le_i32_e64 %[m2], %[a2], %[x]\n"
+            "v_cmp_le_i32_e64 %[m3], %[a3], %[x]\n"
+            "v_cndmask_b32_e64 %[r], %[s], %[j0], %[m0]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j1], %[m1]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j2], %[m2]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j3], %[m3]\n"
+            : [r] "=v"(r), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3)
+            : [s] "v"(seed), [a0] "v"(a0), [a1] "v"(a1), [a2] "v"(a2), [a3] "v"(a3),
+              [x] "v"(x), [j0] "I"(J0), [j1] "I"(J0 + 1), [j2] "I"(J0 + 2), [j3] "I"(J0 + 3));
+    else
+        asm volatile(
+            "v_cmp_lt_i32_e64 %[m0], %[a0], %[x]\n"
+            "v_cmp_lt_i32_e64 %[m1], %[a1], %[x]\n"
+            "v_cmp_lt_i32_e64 %[m2], %[a2], %[x]\n"
+            "v_cmp_lt_i32_e64 %[m3], %[a3], %[x]\n"
+            "v_cndmask_b32_e64 %[r], %[s], %[j0], %[m0]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j1], %[m1]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j2], %[m2]\n"
+            "v_cndmask_b32_e64 %[r], %[r], %[j3], %[m3]\n"
+            : [r] "=v"(r), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3)
+            : [s] "v"(seed), [a0] "v"(a0), [a1] "v"(a1), [a2] "v"(a2), [a3] "v"(a3),
+              [x] "v"(x), [j0] "I"(J0), [j1] "I"(J0 + 1), [j2] "I"(J0 + 2), [j3] "I"(J0 + 3));
+    return r;
+}
+
 // r = the last j in [J0, i) with qual(v[j], v[i]), else r (j as an inline
 // constant): blocks of four, then the last one to three as plain selects.
 template <bool INCL, int J0>
@@ -481,7 +527,12 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
 #pragma unroll
     for (int i = 1; i < K; i++) {
         int r = none;
-        inlane_select<INCL, 0>(r, v, i);
+        if (i >= 4) {
+            r = sel4_from<INCL, 0>(none, v[0], v[1], v[2], v[3], v[i]);
+            inlane_select<INCL, 4>(r, v, i);
+        } else {
+            inlane_select<INCL, 0>(r, v, i);
+        }
         res[i] = base + r;
     }
     int lmin = v[0];
@@ -521,11 +572,24 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     }
     __syncthreads();
 
+    // mins of the earlier and of the later waves: uniform, so they are
+    // folded in scalar registers
     int cross = INF, after = INF;
+    {
+        int wm[8];
 #pragma unroll
-    for (int w = 0; w < NWAVE; w++) {
-        if (w < wave) cross = min(cross, sm.wmin[w]);
-        if (w > wave) after = min(after, sm.wmin[w]);
+        for (int h = 0; h < NWAVE / 4; h++) {
+            const int4 a = *reinterpret_cast<const int4*>(&sm.wmin[4 * h]);
+            wm[4 * h] = __builtin_amdgcn_readfirstlane(a.x);
+            wm[4 * h + 1] = __builtin_amdgcn_readfirstlane(a.y);
+            wm[4 * h + 2] = __builtin_amdgcn_readfirstlane(a.z);
+            wm[4 * h + 3] = __builtin_amdgcn_readfirstlane(a.w);
+        }
+#pragma unroll
+        for (int w = 0; w < NWAVE; w++) {
+            if (w < wave) cross = min(cross, wm[w]);
+            if (w > wave) after = min(after, wm[w]);
+        }
     }
     // exclusive prefix-min of the tile's earlier lanes; meaningful only
     // when has_prev (INCL needs the distinction: INF is a legal key)
@@ -610,10 +674,10 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     // one scan for both counts: queries (low 16 bits), open (high 16 bits)
     const int cq = __builtin_popcount(qmask), co = __builtin_popcount(omask);
     int packed = cq | (co << 16);
-    packed += dpp<DPP_ROW_SHR + 1>(0, packed);
-    packed += dpp<DPP_ROW_SHR + 2>(0, packed);
-    packed += dpp<DPP_ROW_SHR + 4>(0, packed);
-    packed += dpp<DPP_ROW_SHR + 8>(0, packed);
+    packed += dpp0<DPP_ROW_SHR + 1>(packed);
+    packed += dpp0<DPP_ROW_SHR + 2>(packed);
+    packed += dpp0<DPP_ROW_SHR + 4>(packed);
+    packed += dpp0<DPP_ROW_SHR + 8>(packed);
     packed += dpp<DPP_ROW_BCAST15, 0xa>(0, packed);
     packed += dpp<DPP_ROW_BCAST31, 0xc>(0, packed);
     const int ptotal = readlane(packed, WAVE - 1);
@@ -622,10 +686,10 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     // spine entries after this lane inside the wave (suffix count)
     const int cs = __builtin_popcount(smask);
     int zs = cs;
-    zs += dpp<DPP_ROW_SHL + 1>(0, zs);
-    zs += dpp<DPP_ROW_SHL + 2>(0, zs);
-    zs += dpp<DPP_ROW_SHL + 4>(0, zs);
-    zs += dpp<DPP_ROW_SHL + 8>(0, zs);  // lane .. row end
+    zs += dpp0<DPP_ROW_SHL + 1>(zs);
+    zs += dpp0<DPP_ROW_SHL + 2>(zs);
+    zs += dpp0<DPP_ROW_SHL + 4>(zs);
+    zs += dpp0<DPP_ROW_SHL + 8>(zs);  // lane .. row end
     const int rt0 = readlane(zs, 0), rt1 = readlane(zs, 16);
     const int rt2 = readlane(zs, 32), rt3 = readlane(zs, 48);
     zs += row == 0 ? rt1 + rt2 + rt3 : row == 1 ? rt2 + rt3 : row == 2 ? rt3 : 0;

@@ -15,6 +15,7 @@ struct apsep_scratch {
     int device;
     int num_cus;
     int nt_cap;
+    int tile;             // elements per tile the arrays are sized for
     void* base;
     int64_t bytes;
     int* mn0;                   // tile mins
@@ -58,7 +59,7 @@ struct apsep_batch_plan {
 namespace apsep {
 
 int level_sizes(int nt, int* sizes);
-int64_t single_scratch_bytes(int64_t n_cap, apsep_algo algo);
+int64_t single_scratch_bytes(int64_t n_cap, apsep_algo algo, int tile);
 apsep_status run_lookback(apsep_scratch* s, const int32_t* in, int32_t* out, int n, int incl,
                           hipStream_t st);
 apsep_status single_scratch_init(apsep_scratch* s);
@@ -67,6 +68,9 @@ apsep_status single_run(apsep_scratch* s, const int32_t* in, int32_t* out, int n
 apsep_status single_error(apsep_scratch* s, int* code);
 apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs,
                      int32_t* out, int nseg, int n, hipStream_t st);
+// Elements per tile of the segmented solver (its own tile shape: the
+// scratch of a batch plan is sized with it).
+int seg_tile_elems();
 
 // One full solve, then `reps` back-to-back launches of each of its kernels
 // (each launch is idempotent), timed per kernel batch with HIP events.
