@@ -75,6 +75,7 @@ constexpr int FAN = 64;            // hierarchy fan-out
 constexpr int LOG_FAN = 6;
 constexpr int MAX_LEVELS = 5;      // 64^5 tiles >> 2^31 / TILE
 constexpr int UPPER_MIN = 512;     // level-1 size above which levels >= 2 are stored
+constexpr int K1_LOG = 5;          // level-1 keys 2^5 u64 (256 B) apart (see key_at)
 constexpr int INF = INT_MAX;
 #define SPT_WAVES_PER_EU 4         // occupancy the persistent kernel is pinned at
 
@@ -100,6 +101,17 @@ struct Levels {
     u64* key[MAX_LEVELS];     // levels >= 1: epoch-keyed mins (key[0] unused)
     unsigned epoch;           // this run's tag (never 0 or ~0u)
 };
+
+// Entry idx of keyed level lvl.  Level-1 keys take one 256-byte line each:
+// the tile pass folds every tile min into its group's key with a device
+// atomic, and atomics on keys sharing a line serialise (packed keys cost the
+// tile pass ~4 us at N = 2^25 on MI355X).  Upper levels are packed.
+__host__ __device__ __forceinline__ size_t key_slot(int lvl, int idx) {
+    return (size_t)idx << (lvl == 1 ? K1_LOG : 0);
+}
+__device__ __forceinline__ u64& key_at(const Levels& L, int lvl, int idx) {
+    return L.key[lvl][key_slot(lvl, idx)];
+}
 
 __device__ __forceinline__ u64 make_key(unsigned epoch, int v) {
     return ((u64)(~epoch) << 32) | (u64)((unsigned)v ^ 0x80000000u);
@@ -258,10 +270,9 @@ __device__ __forceinline__ int scan_back(const int* __restrict__ arr, int lo,
 
 // Same over an epoch-keyed level.
 template <bool INCL>
-__device__ __forceinline__ int scan_back_key(const u64* __restrict__ arr, int lo, int hi,
-                                             int v, unsigned epoch) {
+__device__ __forceinline__ int scan_back_key(const Levels& L, int lvl, int lo, int hi, int v) {
     for (int i = hi; i >= lo; i--)
-        if (qual<INCL>(key_value(arr[i], epoch), v)) return i;
+        if (qual<INCL>(key_value(key_at(L, lvl, i), L.epoch), v)) return i;
     return -1;
 }
 
@@ -295,7 +306,7 @@ __device__ int find_tile(const Levels& L, int t, int v) {
     int lvl = 1;
     if (L.upper) {
         while (true) {
-            found = scan_back_key<INCL>(L.key[lvl], (x - 1) & ~(FAN - 1), x - 1, v, L.epoch);
+            found = scan_back_key<INCL>(L, lvl, (x - 1) & ~(FAN - 1), x - 1, v);
             if (found >= 0) break;
             x = (x - 1) >> LOG_FAN;
             lvl++;
@@ -304,11 +315,10 @@ __device__ int find_tile(const Levels& L, int t, int v) {
         while (lvl > 1) {
             lvl--;
             const int lo = found << LOG_FAN;
-            found = scan_back_key<INCL>(L.key[lvl], lo, min(lo + FAN - 1, L.n[lvl] - 1), v,
-                                        L.epoch);
+            found = scan_back_key<INCL>(L, lvl, lo, min(lo + FAN - 1, L.n[lvl] - 1), v);
         }
     } else {
-        found = scan_back_key<INCL>(L.key[1], 0, x - 1, v, L.epoch);
+        found = scan_back_key<INCL>(L, 1, 0, x - 1, v);
         if (found < 0) return -1;
     }
     const int lo = found << LOG_FAN;
@@ -855,7 +865,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         L.mn0[t] = m;
         to.qcount[t] = total;
         to.scount[t] = stotal;
-        if (L1 && L.nlev > 1) atomicMin(&L.key[1][t >> LOG_FAN], make_key(L.epoch, m));
+        if (L1 && L.nlev > 1) atomicMin(&key_at(L, 1, t >> LOG_FAN), make_key(L.epoch, m));
     }
     __syncthreads();  // sm is reused by the next tile
 }
@@ -935,7 +945,7 @@ __device__ __forceinline__ void build_level1(const Levels& L) {
     if (g >= L.n[1]) return;  // uniform per wave
     const int idx = g * FAN + lane;
     const int m = wave_min(idx < L.nt ? L.mn0[idx] : INF);
-    if (lane == 0) L.key[1][g] = make_key(L.epoch, m);
+    if (lane == 0) key_at(L, 1, g) = make_key(L.epoch, m);
 }
 
 __device__ __forceinline__ void build_upper_levels(const Levels& L) {
@@ -944,9 +954,9 @@ __device__ __forceinline__ void build_upper_levels(const Levels& L) {
     for (int l = 2; l < L.nlev; l++) {
         for (int g = wv; g < L.n[l]; g += nwv) {
             const int idx = g * FAN + lane;
-            const int x = idx < L.n[l - 1] ? key_value(L.key[l - 1][idx], L.epoch) : INF;
+            const int x = idx < L.n[l - 1] ? key_value(key_at(L, l - 1, idx), L.epoch) : INF;
             const int m = wave_min(x);
-            if (lane == 0) L.key[l][g] = make_key(L.epoch, m);
+            if (lane == 0) key_at(L, l, g) = make_key(L.epoch, m);
         }
         __syncthreads();
     }
@@ -956,7 +966,7 @@ __device__ __forceinline__ void build_upper_levels(const Levels& L) {
 // every level step is one coalesced 64-entry load and a ballot, so an answer
 // arbitrarily far back costs a few rounds instead of a per-lane walk.
 __device__ __forceinline__ int level_entry(const Levels& L, int lvl, int idx) {
-    return lvl == 0 ? L.mn0[idx] : key_value(L.key[lvl][idx], L.epoch);
+    return lvl == 0 ? L.mn0[idx] : key_value(key_at(L, lvl, idx), L.epoch);
 }
 
 template <bool INCL>
@@ -1118,8 +1128,8 @@ __device__ __forceinline__ int range_min_lane(const Levels& L, int a, int b) {
             if (a + lane < ga) e0a = L.mn0[a + lane];
             if (gb + lane < b) e0b = L.mn0[gb + lane];
         } else {
-            if (a + lane < ga) ek[lvl][0] = L.key[lvl][a + lane];
-            if (gb + lane < b) ek[lvl][1] = L.key[lvl][gb + lane];
+            if (a + lane < ga) ek[lvl][0] = key_at(L, lvl, a + lane);
+            if (gb + lane < b) ek[lvl][1] = key_at(L, lvl, gb + lane);
         }
         a = ga >> LOG_FAN;
         b = gb >> LOG_FAN;
@@ -1183,7 +1193,7 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     u64 k1raw = 0;  // (a key of no run: reads as INF)
     int mw2 = INF;
     if (two) {
-        if (lane < L.n[1]) k1raw = L.key[1][lane];
+        if (lane < L.n[1]) k1raw = key_at(L, 1, lane);
         if (t > WAVE && gs + lane < t - WAVE) mw2 = L.mn0[gs + lane];
     }
     const int k1 = key_value(k1raw, L.epoch);
@@ -1204,18 +1214,18 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     } else if (L.nlev > 1) {
         const int a1 = t >> LOG_FAN;
         const int b1 = (a1 >> LOG_FAN) << LOG_FAN;
-        if (b1 + lane < a1) p = min(p, key_value(L.key[1][b1 + lane], L.epoch));
+        if (b1 + lane < a1) p = min(p, key_value(key_at(L, 1, b1 + lane), L.epoch));
         if (L.upper) {
 #pragma unroll
             for (int l = 2; l < MAX_LEVELS; l++) {
                 if (l < L.nlev) {
                     const int a = t >> (LOG_FAN * l);
                     const int idx = ((a >> LOG_FAN) << LOG_FAN) + lane;
-                    if (idx < a) p = min(p, key_value(L.key[l][idx], L.epoch));
+                    if (idx < a) p = min(p, key_value(key_at(L, l, idx), L.epoch));
                 }
             }
         } else {
-            for (int i = lane; i < b1; i += WAVE) p = min(p, key_value(L.key[1][i], L.epoch));
+            for (int i = lane; i < b1; i += WAVE) p = min(p, key_value(key_at(L, 1, i), L.epoch));
         }
     }
     const int P = SEG ? seg_prefix_min(in, to, L, s0, t) : wave_min(p);

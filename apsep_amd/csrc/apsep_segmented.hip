@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__
                                                    u64* __restrict__ key1, int nkey1) {
     const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int stride = (int)(gridDim.x * blockDim.x);
-    for (int i = gid; i < nkey1; i += stride) key1[i] = ~0ull;
+    for (int i = gid; i < nkey1; i += stride) key1[key_slot(1, i)] = ~0ull;
     const long long cap = (long long)nt * TILE;
     for (int k = gid; k < nseg; k += stride) {
         const long long s = max(offs[k], 0ll);

@@ -103,7 +103,7 @@ int64_t single_scratch_bytes(int64_t n_cap, apsep_algo algo, int tile) {
     int sz[MAX_LEVELS];
     const int nl = level_sizes(nt < 1 ? 1 : nt, sz);
     int64_t b = (round4(sz[0]) + 4) * 4;             // tile mins
-    for (int l = 1; l < nl; l++) b += (round4(sz[l]) + 4) * 8;  // keyed levels
+    for (int l = 1; l < nl; l++) b += ((int64_t)key_slot(l, round4(sz[l]) + 4)) * 8;  // keyed levels
     b += (round4((int64_t)nt * cpt) + 4) * 4;       // cmin
     b += (round4(nt) + 4) * 4;                        // qcount
     b += ((int64_t)nt * QS + 2) * 8;                  // qlist
@@ -141,7 +141,7 @@ apsep_status single_scratch_init(apsep_scratch* s) {
     }
     char* p = (char*)s->base;
     s->mn0 = (int*)carve(p, (round4(sz[0]) + 4) * 4);
-    for (int l = 1; l < nl; l++) s->key[l] = (u64*)carve(p, (round4(sz[l]) + 4) * 8);
+    for (int l = 1; l < nl; l++) s->key[l] = (u64*)carve(p, (int64_t)key_slot(l, round4(sz[l]) + 4) * 8);
     s->epoch = 1;
     s->cmin = (int*)carve(p, (round4((int64_t)nt * cpt) + 4) * 4);
     s->qcount = (int*)carve(p, (round4(nt) + 4) * 4);
@@ -163,7 +163,7 @@ apsep_status single_scratch_init(apsep_scratch* s) {
     }
     // keys start above every real key (a fresh scratch holds no run)
     for (int l = 1; l < nl; l++)
-        if (hipMemset(s->key[l], 0xFF, (size_t)(round4(sz[l]) + 4) * 8) != hipSuccess) {
+        if (hipMemset(s->key[l], 0xFF, key_slot(l, round4(sz[l]) + 4) * 8) != hipSuccess) {
             (void)hipGetLastError();
             return APSEP_ERR_LAUNCH;
         }
@@ -208,7 +208,7 @@ static apsep_status fill_levels(apsep_scratch* s, int nt, Levels& L, hipStream_t
         int szc[MAX_LEVELS];
         const int nlc = level_sizes(s->nt_cap, szc);
         for (int l = 1; l < nlc; l++)
-            if (hipMemsetAsync(s->key[l], 0xFF, (size_t)szc[l] * 8, st) != hipSuccess)
+            if (hipMemsetAsync(s->key[l], 0xFF, key_slot(l, szc[l]) * 8, st) != hipSuccess)
                 return APSEP_ERR_LAUNCH;
         s->epoch = 1;
     }
