@@ -38,6 +38,7 @@ EXPORTS = [
     "apsep_scratch_check", "apsep_time_kernels",
     "apsep_batch_plan_create", "apsep_batch_plan_destroy", "apsep_run_segmented",
     "apsep_batch_plan_use_graph", "apsep_batch_plan_tune",
+    "apsep_batch_plan_set_tile_lanes", "apsep_batch_plan_tile_lanes",
     "apsep_server_create", "apsep_server_destroy", "apsep_server_submit",
     "apsep_server_poll", "apsep_server_job_info", "apsep_server_release",
     "apsep_server_drain", "apsep_server_submit_many", "apsep_server_done_upto",
@@ -105,6 +106,8 @@ def lib() -> ctypes.CDLL:
         L.apsep_batch_plan_destroy.argtypes = [vp]
         L.apsep_run_segmented.argtypes = [vp, vp, vp, vp, i32, i64, vp]
         L.apsep_batch_plan_use_graph.argtypes = [vp, ci]
+        L.apsep_batch_plan_set_tile_lanes.argtypes = [vp, i32]
+        L.apsep_batch_plan_tile_lanes.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
         L.apsep_batch_plan_tune.argtypes = [vp, vp, vp, vp, i32, i64, vp, i32,
                                             ctypes.POINTER(ctypes.c_float),
                                             ctypes.POINTER(ctypes.c_float)]
@@ -243,9 +246,22 @@ class BatchPlan:
         _check(lib().apsep_batch_plan_use_graph(self._h, int(enable)),
                "apsep_batch_plan_use_graph")
 
+    def set_tile_lanes(self, lanes: int):
+        """Tile shape of this plan's batches: 256 or 512 lanes x 16 keys."""
+        _check(lib().apsep_batch_plan_set_tile_lanes(self._h, int(lanes)),
+               "apsep_batch_plan_set_tile_lanes")
+
+    @property
+    def tile_lanes(self) -> int:
+        v = ctypes.c_int32(0)
+        _check(lib().apsep_batch_plan_tile_lanes(self._h, ctypes.byref(v)),
+               "apsep_batch_plan_tile_lanes")
+        return v.value
+
     def tune(self, x, offsets, out, total: int | None = None, stream=None, reps: int = 3):
-        """Time direct launches against graph replay on this batch and keep
-        the faster; returns (eager_ms, graph_ms)."""
+        """Time both tile shapes, each by direct launches and by graph
+        replay, on this batch and keep the fastest; returns the chosen
+        shape's (eager_ms, graph_ms)."""
         import torch
         if total is None:
             total = x.numel()

@@ -197,7 +197,10 @@ apsep_status apsep_batch_plan_create(int64_t total_cap, int32_t nseg_cap, int in
     if (!p) return APSEP_ERR_ALLOC;
     p->total_cap = total_cap;
     p->nseg_cap = nseg_cap;
-    const int tile = seg_tile_elems();
+    // sized for the smaller tile, which covers both shapes: as many tile
+    // slots or more, the same chunk-min count (one per 64 keys)
+    const int tile = seg256::seg_tile_elems();
+    p->tile_lanes = 256;  // the faster shape on long batches (tune picks per batch)
     apsep_status st = scratch_create(APSEP_ALGO_WSTL, total_cap, incl, tile, &p->s);
     if (st != APSEP_OK) {
         delete p;
@@ -233,6 +236,25 @@ apsep_status apsep_batch_plan_destroy(apsep_batch_plan* p) {
     return APSEP_OK;
 }
 
+static apsep_status plan_run(apsep_batch_plan* p, const int32_t* d_in, const int64_t* offs,
+                             int32_t* d_out, int nseg, int total, hipStream_t st) {
+    return p->tile_lanes == 512 ? seg512::seg_run(p, d_in, offs, d_out, nseg, total, st)
+                                : seg256::seg_run(p, d_in, offs, d_out, nseg, total, st);
+}
+
+apsep_status apsep_batch_plan_set_tile_lanes(apsep_batch_plan* p, int32_t lanes) {
+    if (!p || (lanes != 256 && lanes != 512)) return APSEP_ERR_INVALID_ARG;
+    if (lanes != p->tile_lanes) drop_graph(p);  // a recorded graph holds the other shape
+    p->tile_lanes = lanes;
+    return APSEP_OK;
+}
+
+apsep_status apsep_batch_plan_tile_lanes(const apsep_batch_plan* p, int32_t* lanes) {
+    if (!p || !lanes) return APSEP_ERR_INVALID_ARG;
+    *lanes = p->tile_lanes;
+    return APSEP_OK;
+}
+
 apsep_status apsep_batch_plan_use_graph(apsep_batch_plan* p, int enable) {
     if (!p || (enable != 0 && enable != 1)) return APSEP_ERR_INVALID_ARG;
     if (!enable) drop_graph(p);
@@ -258,7 +280,7 @@ apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
     }
     hipStream_t hs = (hipStream_t)stream;
     if (!p->use_graph || !stream)
-        return seg_run(p, d_in, d_offsets, d_out, nseg, (int)total, hs);
+        return plan_run(p, d_in, d_offsets, d_out, nseg, (int)total, hs);
     const bool same = p->exec && p->g_in == d_in && p->g_offs == d_offsets &&
                       p->g_out == d_out && p->g_nseg == nseg && p->g_total == total &&
                       p->g_stream == stream;
@@ -268,7 +290,7 @@ apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
             (void)hipGetLastError();
             return APSEP_ERR_LAUNCH;
         }
-        st = seg_run(p, d_in, d_offsets, d_out, nseg, (int)total, hs);
+        st = plan_run(p, d_in, d_offsets, d_out, nseg, (int)total, hs);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(hs, &g);
         if (st != APSEP_OK || ec != hipSuccess || !g) {
@@ -313,19 +335,25 @@ apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
         (void)hipEventDestroy(e0);
         return APSEP_ERR_DEVICE;
     }
-    float t[2] = {0.f, 0.f};
+    // every tile shape x launch mode, timed on this batch
+    const int32_t shapes[2] = {256, 512};
+    float t[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
     apsep_status st = APSEP_OK;
-    for (int g = 0; g < 2 && st == APSEP_OK; g++) {
-        st = apsep_batch_plan_use_graph(p, g);
-        // one untimed call (records the graph), then the timed ones
-        if (st == APSEP_OK) st = apsep_run_segmented(p, d_in, d_offsets, d_out, nseg, total, stream);
-        if (st == APSEP_OK && hipEventRecord(e0, hs) != hipSuccess) st = APSEP_ERR_LAUNCH;
-        for (int r = 0; r < reps && st == APSEP_OK; r++)
-            st = apsep_run_segmented(p, d_in, d_offsets, d_out, nseg, total, stream);
-        if (st == APSEP_OK &&
-            (hipEventRecord(e1, hs) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-             hipEventElapsedTime(&t[g], e0, e1) != hipSuccess))
-            st = APSEP_ERR_LAUNCH;
+    for (int k = 0; k < 2 && st == APSEP_OK; k++) {
+        st = apsep_batch_plan_set_tile_lanes(p, shapes[k]);
+        for (int g = 0; g < 2 && st == APSEP_OK; g++) {
+            st = apsep_batch_plan_use_graph(p, g);
+            // one untimed call (records the graph), then the timed ones
+            if (st == APSEP_OK)
+                st = apsep_run_segmented(p, d_in, d_offsets, d_out, nseg, total, stream);
+            if (st == APSEP_OK && hipEventRecord(e0, hs) != hipSuccess) st = APSEP_ERR_LAUNCH;
+            for (int r = 0; r < reps && st == APSEP_OK; r++)
+                st = apsep_run_segmented(p, d_in, d_offsets, d_out, nseg, total, stream);
+            if (st == APSEP_OK &&
+                (hipEventRecord(e1, hs) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                 hipEventElapsedTime(&t[k][g], e0, e1) != hipSuccess))
+                st = APSEP_ERR_LAUNCH;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -333,9 +361,16 @@ apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
         (void)hipGetLastError();
         return st;
     }
-    if (eager_ms) *eager_ms = t[0] / reps;
-    if (graph_ms) *graph_ms = t[1] / reps;
-    return apsep_batch_plan_use_graph(p, t[1] < t[0] ? 1 : 0);
+    int bk = 0, bg = 0;
+    for (int k = 0; k < 2; k++)
+        for (int g = 0; g < 2; g++)
+            if (t[k][g] < t[bk][bg]) bk = k, bg = g;
+    if (eager_ms) *eager_ms = t[bk][0] / reps;
+    if (graph_ms) *graph_ms = t[bk][1] / reps;
+    st = apsep_batch_plan_set_tile_lanes(p, shapes[bk]);
+    if (st != APSEP_OK) return st;
+    // d_out holds the last shape's answer: bit-identical across shapes
+    return apsep_batch_plan_use_graph(p, bg);
 }
 
 apsep_status apsep_launch(apsep_algo algo, const int32_t* d_in, int32_t* d_out,

@@ -44,6 +44,7 @@ struct apsep_batch_plan {
     int32_t nseg_cap;
     int* tmap;             // [tiles at capacity]
     int2* tbound;          // [tiles at capacity]: that segment's start and end
+    int tile_lanes;        // tile shape the batches run with: 256 or 512 lanes
     int use_graph;
     hipGraph_t graph;
     hipGraphExec_t exec;
@@ -66,11 +67,19 @@ apsep_status single_scratch_init(apsep_scratch* s);
 apsep_status single_run(apsep_scratch* s, const int32_t* in, int32_t* out, int n,
                         int incl, hipStream_t st);
 apsep_status single_error(apsep_scratch* s, int* code);
+// The segmented solver, built for tiles of 256 and of 512 lanes
+// (apsep_segmented_256.hip, apsep_segmented_512.hip).  seg_tile_elems is
+// that build's tile in elements.
+namespace seg256 {
 apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs,
                      int32_t* out, int nseg, int n, hipStream_t st);
-// Elements per tile of the segmented solver (its own tile shape: the
-// scratch of a batch plan is sized with it).
 int seg_tile_elems();
+}  // namespace seg256
+namespace seg512 {
+apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs,
+                     int32_t* out, int nseg, int n, hipStream_t st);
+int seg_tile_elems();
+}  // namespace seg512
 
 // One full solve, then `reps` back-to-back launches of each of its kernels
 // (each launch is idempotent), timed per kernel batch with HIP events.

@@ -1,6 +1,12 @@
-// apsep_segmented.hip -- segmented (CSR batch) APSEP for gfx950: many
+// apsep_segmented_impl.hpp -- segmented (CSR batch) APSEP for gfx950: many
 // independent arrays solved by one launch sequence, with optional hipGraph
 // capture and replay of that sequence.
+//
+// Compiled once per tile shape: apsep_segmented_256.hip and
+// apsep_segmented_512.hip define APSEP_TILE_LANES and APSEP_SEG_NS (the
+// namespace that keeps the two builds' kernels and entry points apart) and
+// include this file.  A batch plan runs one of them; apsep_batch_plan_tune
+// times both on the caller's batch.
 //
 // The reference has no batched entry point; its callers loop over arrays
 // with one launchWSTL / launchSPT each (src/apsep.cuh:2598-2605,
@@ -16,12 +22,9 @@
 // The level-1 keys are reset by the setup kernel (a kernel, not a memset,
 // so the sequence can be captured into a graph), and the run uses a fixed
 // epoch: every replay of a captured graph then starts from clean keys.
-// The segmented solver's tiles are 256 lanes x 16 elements: over a long
-// batch, workgroups of four waves overlap their load, solve and store phases
-// better than the single-array solver's eight-wave ones (1.55 instead of
-// 1.69 ms for 8192 segments / 8.7e8 keys on MI355X).  Its batch plans size
-// their scratch with this tile (seg_tile_elems).
-#define APSEP_TILE_LANES 256
+#if !defined(APSEP_TILE_LANES) || !defined(APSEP_SEG_NS)
+#error "include through apsep_segmented_256.hip / apsep_segmented_512.hip"
+#endif
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,6 +36,7 @@
 #include "../../include/apsep/apsep.h"
 
 namespace apsep {
+namespace APSEP_SEG_NS {
 
 int seg_tile_elems() { return TILE; }
 
@@ -126,4 +130,5 @@ apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs
                       : seg_launch<false>(p, in, o, out, nseg, n, st);
 }
 
+}  // namespace APSEP_SEG_NS
 }  // namespace apsep

@@ -97,14 +97,24 @@ int level_sizes(int nt, int* sizes) {
     return nl;
 }
 
+// Chunk-min slots: whole tiles of the scratch's own shape and of the
+// largest shape (this file's tile): a batch plan's scratch, sized for the
+// smaller segmented tile, also serves batches run on the larger one, whose
+// last partial tile writes all of its chunk mins.
+static int64_t chunk_slots(int64_t n_cap, int tile) {
+    const int64_t own = (n_cap + tile - 1) / tile * (tile / CHUNK);
+    const int64_t big = (n_cap + TILE - 1) / TILE * CPT;
+    return own > big ? own : big;
+}
+
 int64_t single_scratch_bytes(int64_t n_cap, apsep_algo algo, int tile) {
     const int nt = (int)((n_cap + tile - 1) / tile);
-    const int cpt = tile / CHUNK;
+    const int64_t ncm = chunk_slots(n_cap, tile);
     int sz[MAX_LEVELS];
     const int nl = level_sizes(nt < 1 ? 1 : nt, sz);
     int64_t b = (round4(sz[0]) + 4) * 4;             // tile mins
     for (int l = 1; l < nl; l++) b += ((int64_t)key_slot(l, round4(sz[l]) + 4)) * 8;  // keyed levels
-    b += (round4((int64_t)nt * cpt) + 4) * 4;       // cmin
+    b += (round4(ncm) + 4) * 4;                       // cmin
     b += (round4(nt) + 4) * 4;                        // qcount
     b += ((int64_t)nt * QS + 2) * 8;                  // qlist
     b += (round4(nt) + 4) * 4;                        // scount
@@ -113,7 +123,7 @@ int64_t single_scratch_bytes(int64_t n_cap, apsep_algo algo, int tile) {
     if (algo == APSEP_ALGO_LOOKBACK) {
         b += (int64_t)sizeof(LbCtl);
         b += 2 * (round4(nt) + 4) * 8;                // tile mins, inclusive prefixes
-        b += (round4((int64_t)nt * cpt) + 4) * 8;      // chunk mins
+        b += (round4(ncm) + 4) * 8;                   // chunk mins
         for (int l = 1; l < nl; l++) b += (round4(sz[l]) + 4) * 8;  // group mins
         b += 256 * (4 + LB_LEVELS);
     }
@@ -128,7 +138,7 @@ static void* carve(char*& p, int64_t bytes) {
 
 apsep_status single_scratch_init(apsep_scratch* s) {
     const int nt = (int)((s->n_cap + s->tile - 1) / s->tile);
-    const int cpt = s->tile / CHUNK;
+    const int64_t ncm = chunk_slots(s->n_cap, s->tile);
     s->nt_cap = nt;
     if (nt == 0) return APSEP_OK;
     int sz[MAX_LEVELS];
@@ -143,7 +153,7 @@ apsep_status single_scratch_init(apsep_scratch* s) {
     s->mn0 = (int*)carve(p, (round4(sz[0]) + 4) * 4);
     for (int l = 1; l < nl; l++) s->key[l] = (u64*)carve(p, (int64_t)key_slot(l, round4(sz[l]) + 4) * 8);
     s->epoch = 1;
-    s->cmin = (int*)carve(p, (round4((int64_t)nt * cpt) + 4) * 4);
+    s->cmin = (int*)carve(p, (round4(ncm) + 4) * 4);
     s->qcount = (int*)carve(p, (round4(nt) + 4) * 4);
     s->qlist = (int2*)carve(p, ((int64_t)nt * QS + 2) * 8);
     s->scount = (int*)carve(p, (round4(nt) + 4) * 4);
@@ -153,7 +163,7 @@ apsep_status single_scratch_init(apsep_scratch* s) {
         s->lb_ctl = (LbCtl*)carve(p, (int64_t)sizeof(LbCtl));
         s->lb_M = (u64*)carve(p, (round4(nt) + 4) * 8);
         s->lb_S = (u64*)carve(p, (round4(nt) + 4) * 8);
-        s->lb_CM = (u64*)carve(p, (round4((int64_t)nt * cpt) + 4) * 8);
+        s->lb_CM = (u64*)carve(p, (round4(ncm) + 4) * 8);
         for (int l = 1; l < nl; l++) s->lb_G[l] = (u64*)carve(p, (round4(sz[l]) + 4) * 8);
     }
     if ((char*)p > (char*)s->base + s->bytes) return APSEP_ERR_ALLOC;  // sizing bug guard

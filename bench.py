@@ -189,8 +189,10 @@ def bench_batch(args, torch, apsep_amd):
     with apsep_amd.BatchPlan(total, nseg) as p:
         run = lambda: p.run(x, offs, out, total=total, stream=stream)
         with torch.cuda.stream(stream):
+            # both tile shapes x direct/graph; the plan keeps the fastest
             tuned = p.tune(x, offs, out, total=total, stream=stream)
             res["tuned_mode"] = "graph" if tuned[1] < tuned[0] else "eager"
+            res["tile_lanes"] = p.tile_lanes
             p.use_graph(False)
             for _ in range(args.warmup):
                 run()
@@ -240,7 +242,7 @@ def bench_batch(args, torch, apsep_amd):
         "config": {"workload": "segmented CSR batch, %d segments, %d keys" % (nseg, total),
                    "segments": nseg, "total": total},
         "eager_ms": round(res["eager_ms"], 3), "graph_ms": round(res["graph_ms"], 3),
-        "autotuned_mode": res["tuned_mode"],
+        "autotuned_mode": res["tuned_mode"], "autotuned_tile_lanes": res["tile_lanes"],
         "per_array_run_ms": round(res["per_array_run_ms"], 2),
         "per_array_launch_ms": round(res["per_array_launch_ms"], 2),
         "speedup_vs_per_array_run": round(res["per_array_run_ms"] / best, 2),

@@ -166,10 +166,17 @@ apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
  * The sequence holds no memset node. */
 apsep_status apsep_batch_plan_use_graph(apsep_batch_plan* p, int enable);
 
-/* Autotune a plan for a batch shape: times `reps` direct-launch runs and
- * `reps` graph replays of this batch on `stream` (non-null), keeps the
- * faster mode and reports both mean times (blocking; d_out holds the
- * answer afterwards). */
+/* Tile shape of a plan's batches: 256 (the default) or 512 lanes of 16
+ * keys per workgroup.  Answers do not depend on it.  No counterpart in the
+ * reference (its kernels take BLOCK_SIZE / IPT as template parameters,
+ * src/apsep.cuh:2548, 3052); set_tile_lanes drops a recorded graph. */
+apsep_status apsep_batch_plan_set_tile_lanes(apsep_batch_plan* p, int32_t lanes);
+apsep_status apsep_batch_plan_tile_lanes(const apsep_batch_plan* p, int32_t* lanes);
+
+/* Autotune a plan for a batch shape: for each tile shape, times `reps`
+ * direct-launch runs and `reps` graph replays of this batch on `stream`
+ * (non-null); keeps the fastest shape and mode and reports that shape's two
+ * mean times (blocking; d_out holds the answer afterwards). */
 apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
                                    const int64_t* d_offsets, int32_t* d_out, int32_t nseg,
                                    int64_t total, void* stream, int32_t reps,

@@ -110,6 +110,10 @@ def test_batch_plan_argument_checks():
     assert L.apsep_batch_plan_destroy(None) == A.OK
     assert L.apsep_batch_plan_use_graph(None, 1) == A.ERR_INVALID_ARG
     assert L.apsep_run_segmented(None, 4096, 8192, 16384, 1, 10, None) == A.ERR_INVALID_ARG
+    assert L.apsep_batch_plan_set_tile_lanes(None, 256) == A.ERR_INVALID_ARG
+    lanes = ctypes.c_int32(7)
+    assert L.apsep_batch_plan_tile_lanes(None, ctypes.byref(lanes)) == A.ERR_INVALID_ARG
+    assert lanes.value == 7
 
 
 def test_server_argument_checks():

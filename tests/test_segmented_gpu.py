@@ -2,7 +2,8 @@
 oracle's per-segment monotone stack (answers relative to each segment
 start).  Covers empty and one-element segments, segments that start inside
 a lane, at lane/chunk/tile edges and span many tiles, every input class,
-both predicates, and hipGraph capture/replay with new data between replays.
+both predicates, both tile shapes (256 and 512 lanes), the autotuner's
+choice, and hipGraph capture/replay with new data between replays.
 """
 import sys
 
@@ -67,20 +68,25 @@ EDGE_LENGTHS = [0, 1, 2, 15, 16, 17, 0, 63, 64, 65, 4095, 4096, 4097, 1, 0, 8191
                 3, 70000, 5, 0, 4096 * 70 + 3, 7, 1]
 
 
+@pytest.mark.parametrize("lanes", [256, 512])
 @pytest.mark.parametrize("incl", [False, True])
 @pytest.mark.parametrize("kind", ["random", "descending", "ascending", "dup2", "mixed"])
-def test_edge_lengths(torch_mod, kind, incl):
+def test_edge_lengths(torch_mod, kind, incl, lanes):
     import apsep_amd as A
     lengths = EDGE_LENGTHS
     a = _keys(kind, lengths, seed=3)
     off = _offsets(lengths)
     with A.BatchPlan(a.shape[0], len(lengths), incl) as p:
-        _log(kind, incl, a.shape[0])
-        _check(_solve(p, torch_mod, a, off), a, off, incl, "edge %s incl=%d" % (kind, incl))
+        p.set_tile_lanes(lanes)
+        assert p.tile_lanes == lanes
+        _log(kind, incl, lanes, a.shape[0])
+        _check(_solve(p, torch_mod, a, off), a, off, incl,
+               "edge %s incl=%d lanes=%d" % (kind, incl, lanes))
 
 
+@pytest.mark.parametrize("lanes", [256, 512])
 @pytest.mark.parametrize("seed", [1, 2, 3])
-def test_many_small_segments(torch_mod, seed):
+def test_many_small_segments(torch_mod, seed, lanes):
     """Thousands of segments shorter than a lane / a tile (many starts per
     tile, several per lane)."""
     import apsep_amd as A
@@ -89,11 +95,14 @@ def test_many_small_segments(torch_mod, seed):
     a = _keys("mixed", lengths, seed)
     off = _offsets(lengths)
     with A.BatchPlan(a.shape[0], len(lengths)) as p:
-        _log("small", seed, len(lengths), a.shape[0])
-        _check(_solve(p, torch_mod, a, off), a, off, False, "small seed=%d" % seed)
+        p.set_tile_lanes(lanes)
+        _log("small", seed, lanes, len(lengths), a.shape[0])
+        _check(_solve(p, torch_mod, a, off), a, off, False,
+               "small seed=%d lanes=%d" % (seed, lanes))
 
 
-def test_loguniform_batch(torch_mod):
+@pytest.mark.parametrize("lanes", [256, 512])
+def test_loguniform_batch(torch_mod, lanes):
     """A scaled-down batched configuration: log-uniform lengths in
     [64, 2^18], class uniform over random/descending/ascending."""
     import apsep_amd as A
@@ -101,12 +110,48 @@ def test_loguniform_batch(torch_mod):
     lengths = np.diff(off)
     a = _keys("mixed", lengths, 9)
     with A.BatchPlan(a.shape[0], len(lengths)) as p:
-        _log("loguniform", len(lengths), a.shape[0])
+        p.set_tile_lanes(lanes)
+        _log("loguniform", lanes, len(lengths), a.shape[0])
         _check(_solve(p, torch_mod, a, off), a, off, False, "loguniform")
         # the same plan again, smaller batch (capacity is an upper bound)
         off2 = off[:301]
         a2 = a[:int(off2[-1])]
         _check(_solve(p, torch_mod, a2, off2), a2, off2, False, "loguniform prefix")
+
+
+def test_tune_picks_a_shape_and_switching_shapes_keeps_answers(torch_mod):
+    """The autotuner times both tile shapes (direct and graph) on the batch
+    and keeps the fastest; its output is the answer.  Switching the shape of
+    a graph plan afterwards drops the recorded graph, and both shapes give
+    the same bits."""
+    import apsep_amd as A
+    torch = torch_mod
+    off = oracle.loguniform_offsets(77, 400, 64.0, float(1 << 17))
+    lengths = np.diff(off)
+    a = _keys("mixed", lengths, 21)
+    total = a.shape[0]
+    x = torch.from_numpy(a).cuda()
+    o = torch.from_numpy(off).cuda()
+    out = torch.full((total,), int(POISON), dtype=torch.int32, device="cuda")
+    stream = torch.cuda.Stream()
+    with A.BatchPlan(total, len(lengths)) as p:
+        _log("tune", len(lengths), total)
+        with torch.cuda.stream(stream):
+            e, g = p.tune(x, o, out, total=total, stream=stream, reps=2)
+        stream.synchronize()
+        assert e > 0 and g > 0
+        chosen = p.tile_lanes
+        assert chosen in (256, 512)
+        _check(out.cpu().numpy(), a, off, False, "after tune (lanes=%d)" % chosen)
+        p.use_graph(True)
+        for lanes in (512, 256, 512):
+            p.set_tile_lanes(lanes)
+            out.fill_(int(POISON))
+            torch.cuda.synchronize()
+            with torch.cuda.stream(stream):
+                p.run(x, o, out, total=total, stream=stream)
+            stream.synchronize()
+            _check(out.cpu().numpy(), a, off, False, "graph plan lanes=%d" % lanes)
 
 
 def test_one_segment_equals_plain_solve(torch_mod):
