@@ -5,16 +5,18 @@
 // (strict) or in[j] <= in[i] (INCL), else -1.
 //
 // Work decomposition (wave64 throughout):
-//   tile   = one workgroup of NT = 512 lanes (8 waves) x K = 16 consecutive
-//            elements per lane = 8192 elements, loaded with four dwordx4
-//            loads per lane.
+//   tile   = one workgroup of NT lanes x K = 16 consecutive elements per
+//            lane: 512 lanes (8 waves, 8192 elements) for single arrays,
+//            256 or 512 for segmented batches (APSEP_TILE_LANES), loaded
+//            with four dwordx4 loads per lane.
 //   chunk  = 64 consecutive elements (4 lanes); its min is published for the
 //            inter-tile search.
 //   levels = 64-ary min hierarchy over tile mins.  Level 0 (tile mins) is a
 //            plain array; levels >= 1 hold 64-bit keys (run epoch in the high
 //            word, biased min in the low word).  The tile pass of a plain
 //            solve accumulates level 1 itself with device-scope atomicMin
-//            (64 tiles per key), with no reset between runs and no memory
+//            (64 tiles per key, keys a cache line apart), with no reset
+//            between runs and no memory
 //            fence: a key from an older run compares larger and decodes as
 //            empty.  Segmented batches build level 1 from the tile mins in
 //            a kernel of its own (build_level1).  Levels >= 2 are
@@ -61,12 +63,12 @@ constexpr int WAVE = 64;
 constexpr int NT = APSEP_TILE_LANES;  // lanes per tile workgroup (256 or 512)
 constexpr int K = 16;              // consecutive elements per lane
 constexpr int K4 = K / 4;          // dwordx4 per lane
-constexpr int TILE = NT * K;       // 8192 elements per tile
+constexpr int TILE = NT * K;       // elements per tile (8192 at 512 lanes)
 constexpr int NWAVE = NT / WAVE;   // 8 waves per tile
 constexpr int CHUNK = 64;          // elements per published chunk min
 constexpr int LPC = CHUNK / K;     // lanes per chunk (4)
 constexpr int LOG_LPC = 2;
-constexpr int CPT = TILE / CHUNK;  // 128 chunk mins per tile
+constexpr int CPT = TILE / CHUNK;  // chunk mins per tile (128 at 512 lanes)
 constexpr int QS = 32;             // recorded open elements per tile
 constexpr int RS = 32;             // recorded right-spine entries per tile
 constexpr int QCAP = 192;          // per-wave LDS queue of cross-lane queries
