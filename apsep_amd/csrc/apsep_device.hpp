@@ -1195,7 +1195,8 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     u64 k1raw = 0;  // (a key of no run: reads as INF)
     int mw2 = INF;
     if (two) {
-        if (lane < L.n[1]) k1raw = key_at(L, 1, lane);
+        // only groups before the tile's own are used (prefix, deep search)
+        if (lane < (t >> LOG_FAN)) k1raw = key_at(L, 1, lane);
         if (t > WAVE && gs + lane < t - WAVE) mw2 = L.mn0[gs + lane];
     }
     const int k1 = key_value(k1raw, L.epoch);
