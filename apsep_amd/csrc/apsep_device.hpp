@@ -517,7 +517,8 @@ __device__ __forceinline__ unsigned qual_bits(int p, const int* v) {
 
 template <bool INCL, bool SEG = false, bool L1 = true>
 __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, const int* v,
-                                           TileSmem& sm, const TileOut& to, const Levels& L,
+                                           int prev_last, TileSmem& sm, const TileOut& to,
+                                           const Levels& L,
                                            const SegLane sl = SegLane{0, 0u}) {
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
@@ -626,6 +627,16 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     }
     qmask &= vmask;
     omask &= vmask;
+    // The tile's first element is open (nothing precedes it in the tile).
+    // When the key just before the tile (prev_last, read by lane 0)
+    // qualifies, that key is its answer: settled here, it leaves the
+    // fix-up's list, and ascending input leaves the fix-up nothing to do.
+    // (The fix-up's early-out stays valid: it tests the first element's
+    // value, which bounds every open value.)
+    if (tid == 0 && t > 0 && qual<INCL>(prev_last, v[0])) {
+        res[0] = tile_start - 1;
+        omask &= ~1u;
+    }
 
     // right spine: strict suffix minima of the tile.  A lane holds some iff
     // its min is below everything after it; the tile's last element has
@@ -881,7 +892,8 @@ __device__ __forceinline__ void tile_step(const int* __restrict__ in, int* __res
                                           const Levels& L) {
     int v[K];
     load_lane(in, n, t, v);
-    solve_tile<INCL, false, L1>(out, n, t, v, sm, to, L);
+    const int pl = (threadIdx.x == 0 && t > 0) ? in[t * TILE - 1] : 0;
+    solve_tile<INCL, false, L1>(out, n, t, v, pl, sm, to, L);
 }
 
 // Segmented tile t: mark the segment starts inside the tile in LDS, give
@@ -898,6 +910,7 @@ __device__ __forceinline__ void tile_step_seg(const int* __restrict__ in, int* _
     const int tile_end = tile_start + min(TILE, n - tile_start);  // no overflow near 2^31
     int v[K];
     load_lane(in, n, t, v);
+    const int pl = (threadIdx.x == 0 && t > 0) ? in[t * TILE - 1] : 0;
     // offsets are clamped so that malformed ones cannot send a read astray
     const int last = max(sg.nseg - 1, 0);
     const int k0 = min(max(sg.tmap[t], 0), last);
@@ -931,7 +944,7 @@ __device__ __forceinline__ void tile_step_seg(const int* __restrict__ in, int* _
 #pragma unroll
     for (int w = 0; w < NWAVE - 1; w++)
         if (w < wave) incoming = max(incoming, sm.wlast[w]);
-    solve_tile<INCL, true, false>(out, n, t, v, sm, to, L, SegLane{incoming, wm});
+    solve_tile<INCL, true, false>(out, n, t, v, pl, sm, to, L, SegLane{incoming, wm});
 }
 
 // ---------------------------------------------------------------------------

@@ -167,7 +167,8 @@ __global__ __launch_bounds__(NT) void k_lookback(const int* __restrict__ in,
     V.tag = V.ctl->epoch;  // set by the preceding one-thread kernel
     int v[K];
     load_lane(in, n, t, v);
-    solve_tile<INCL>(out, n, t, v, sm, to, L);  // ends with a workgroup barrier
+    const int pl = (threadIdx.x == 0 && t > 0) ? in[t * TILE - 1] : 0;
+    solve_tile<INCL>(out, n, t, v, pl, sm, to, L);  // ends with a workgroup barrier
     if (threadIdx.x >= WAVE) return;
     const int lane = threadIdx.x;
     const int tile_start = t * TILE;
