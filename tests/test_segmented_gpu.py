@@ -154,6 +154,25 @@ def test_tune_picks_a_shape_and_switching_shapes_keeps_answers(torch_mod):
             _check(out.cpu().numpy(), a, off, False, "graph plan lanes=%d" % lanes)
 
 
+@pytest.mark.parametrize("lanes", [256, 512])
+@pytest.mark.parametrize("cap", [3 * 8192 + 1, 5 * 8192 + 4095, 7 * 4096 + 17])
+def test_capacity_off_tile_multiples(torch_mod, cap, lanes):
+    """Plans whose capacity is not a multiple of either tile, filled to
+    capacity: the last partial tile of the larger shape writes all of its
+    chunk mins, which the plan's scratch (sized for the smaller tile) must
+    hold.  An overrun would corrupt the tile counts stored after them."""
+    import apsep_amd as A
+    lengths = [cap // 3, cap // 3, cap - 2 * (cap // 3)]
+    for kind in ("random", "ascending", "mixed"):
+        a = _keys(kind, lengths, seed=cap % 97)
+        off = _offsets(lengths)
+        with A.BatchPlan(cap, len(lengths)) as p:
+            p.set_tile_lanes(lanes)
+            _log("capacity", cap, lanes, kind)
+            _check(_solve(p, torch_mod, a, off), a, off, False,
+                   "cap=%d lanes=%d %s" % (cap, lanes, kind))
+
+
 def test_one_segment_equals_plain_solve(torch_mod):
     import apsep_amd as A
     n = (1 << 22) + 77
