@@ -44,7 +44,8 @@ constexpr unsigned SEG_EPOCH = 1u;
 constexpr int SEG_FIX_WG = 256;  // four tiles per group, up to 8 waves per SIMD (as k_fix_pass)
 
 // tile -> segment holding the tile's first element, and that segment's
-// start and end; level-1 keys emptied
+// start and end (one wave per segment, its lanes over the segment's tiles:
+// a segment of 2^20 keys covers 256 tiles); level-1 keys emptied
 __global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__ offs,
                                                    int nseg, int* __restrict__ tmap,
                                                    int2* __restrict__ tbound, int nt,
@@ -52,16 +53,18 @@ __global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__
     const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int stride = (int)(gridDim.x * blockDim.x);
     for (int i = gid; i < nkey1; i += stride) key1[key_slot(1, i)] = ~0ull;
+    const int lane = (int)threadIdx.x & (WAVE - 1);
     const long long cap = (long long)nt * TILE;
-    for (int k = gid; k < nseg; k += stride) {
+    for (int k = gid >> 6; k < nseg; k += stride >> 6) {  // uniform per wave
         const long long s = max(offs[k], 0ll);
         const long long e = min(offs[k + 1], cap);
         if (s >= e) continue;
         const int t0 = (int)((s + TILE - 1) / TILE);
         const int t1 = (int)((e - 1) / TILE);
-        for (int t = t0; t <= t1; t++) {
+        const int2 b = make_int2((int)min(s, (long long)INT_MAX), (int)min(e, (long long)INT_MAX));
+        for (int t = t0 + lane; t <= t1; t += WAVE) {
             tmap[t] = k;
-            tbound[t] = make_int2((int)min(s, (long long)INT_MAX), (int)min(e, (long long)INT_MAX));
+            tbound[t] = b;
         }
     }
 }
@@ -75,7 +78,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
 }
 
 __global__ __launch_bounds__(256) void k_level1_seg(Levels L) { build_level1(L); }
-__global__ __launch_bounds__(256) void k_upper_levels_seg(Levels L) { build_upper_levels(L); }
+// level lvl >= 2 from the level below, one wave per entry (one launch per
+// level: each reads the whole level below)
+__global__ __launch_bounds__(256) void k_level_seg(Levels L, int lvl) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int g = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (g >= L.n[lvl]) return;  // uniform per wave
+    const int idx = g * FAN + lane;
+    const int x = idx < L.n[lvl - 1] ? key_value(key_at(L, lvl - 1, idx), L.epoch) : INF;
+    const int m = wave_min(x);
+    if (lane == 0) key_at(L, lvl, g) = make_key(L.epoch, m);
+}
 
 template <bool INCL>
 __global__ __launch_bounds__(SEG_FIX_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_fix_pass_seg(const int* __restrict__ in,
@@ -106,16 +119,19 @@ static apsep_status seg_launch(apsep_batch_plan* p, const int* in, const long lo
     TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
     SegIn sg{offs, p->tmap, nseg, p->tbound};
     const int nkey1 = L.nlev > 1 ? sz[1] : 0;
-    const int work = nseg > nkey1 ? nseg : nkey1;
-    int blocks = (work + 255) / 256;
+    // a wave per segment (4 per block), a thread per level-1 key
+    const long long work = nseg * 64ll > nkey1 ? nseg * 64ll : nkey1;
+    int blocks = (int)((work + 255) / 256);
     if (blocks < 1) blocks = 1;
-    if (blocks > 1024) blocks = 1024;
+    if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_seg_setup, dim3(blocks), dim3(256), 0, st, offs, nseg, p->tmap,
                        p->tbound, nt, nkey1 ? s->key[1] : nullptr, nkey1);
     hipLaunchKernelGGL(k_tile_pass_seg<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L, sg);
     if (nt > 1) {
         if (L.nlev > 1) hipLaunchKernelGGL(k_level1_seg, dim3((L.n[1] + 3) / 4), dim3(256), 0, st, L);
-        if (L.upper) hipLaunchKernelGGL(k_upper_levels_seg, dim3(1), dim3(256), 0, st, L);
+        if (L.upper)
+            for (int l = 2; l < L.nlev; l++)
+                hipLaunchKernelGGL(k_level_seg, dim3((L.n[l] + 3) / 4), dim3(256), 0, st, L, l);
         constexpr int tpw = SEG_FIX_WG / WAVE;
         hipLaunchKernelGGL(k_fix_pass_seg<INCL>, dim3((nt + tpw - 1) / tpw), dim3(SEG_FIX_WG), 0,
                            st, in, out, n, L, to, sg);

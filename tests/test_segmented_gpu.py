@@ -173,6 +173,28 @@ def test_capacity_off_tile_multiples(torch_mod, cap, lanes):
                    "cap=%d lanes=%d %s" % (cap, lanes, kind))
 
 
+@pytest.mark.parametrize("lanes", [256, 512])
+def test_deep_hierarchy_batch(torch_mod, lanes):
+    """2^28 + 12345 keys in 64 segments: level 1 holds more than 512 entries
+    with either tile shape, so levels >= 2 are built (one launch per level)
+    and the fix-up's prefix-mins and searches walk them."""
+    import apsep_amd as A
+    total = (1 << 28) + 12345
+    rng = np.random.default_rng(5)
+    cuts = np.unique(rng.integers(1, total, size=63))
+    off = np.concatenate([[0], cuts, [total]]).astype(np.int64)
+    a = _keys("mixed", np.diff(off), 77)
+    with A.BatchPlan(total, len(off) - 1) as p:
+        p.set_tile_lanes(lanes)
+        _log("deep hierarchy", lanes, total)
+        got = _solve(p, torch_mod, a, off)
+    exp = oracle.apsep_segmented(a, off, False, threads=8)
+    if not np.array_equal(got, exp):
+        i = int(np.nonzero(got != exp)[0][0])
+        raise AssertionError("deep hierarchy lanes=%d: first mismatch i=%d got=%d exp=%d"
+                             % (lanes, i, got[i], exp[i]))
+
+
 def test_one_segment_equals_plain_solve(torch_mod):
     import apsep_amd as A
     n = (1 << 22) + 77
