@@ -620,8 +620,11 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     for (int i = K - 1; i >= 0; i--) rec = __builtin_amdgcn_alignbit(rec, (unsigned)res[i], 31);
     unsigned qmask = rec, omask = 0;
     if (!has_prev || !qual<INCL>(pmx, lmin)) {
+        // v[0] is the largest of them: when pmx does not beat it, it beats
+        // none (every lane of descending input), and the per-element
+        // split is skipped
         unsigned beaten = 0;
-        if (has_prev) beaten = qual_bits<INCL>(pmx, v);
+        if (has_prev && qual<INCL>(pmx, v[0])) beaten = qual_bits<INCL>(pmx, v);
         qmask = rec & beaten;
         omask = rec & ~beaten;
     }
