@@ -62,6 +62,40 @@ def test_batch_of_jobs_all_correct(torch_mod):
             s.release(jid)
 
 
+def test_result_alone_equals_result_in_mix(torch_mod):
+    """Each job served alone (a server with nothing else) and served among
+    others (submitted between them, batched with them) gives the same
+    answer bits, and both are the oracle's."""
+    import apsep_amd as A
+    torch = torch_mod
+    rng = np.random.default_rng(4)
+    lengths = [int(x) for x in rng.integers(1, 300000, size=24)]
+    host, dev = _jobs(torch, lengths, 4)
+    torch.cuda.synchronize()
+    probe = [0, 5, 11, 23]
+    alone = {}
+    for k in probe:
+        with A.Server(1 << 28, 1 << 22, 8) as s:
+            jid = s.submit(dev[k].data_ptr(), lengths[k])
+            s.drain()
+            alone[k] = _answers(s, jid)
+            s.release(jid)
+    with A.Server(1 << 30, 1 << 24, 16) as s:
+        ids = []
+        for k, n in enumerate(lengths):
+            ids.append(s.submit(dev[k].data_ptr(), n))
+            if k % 3 == 0:
+                s.poll()
+        s.drain()
+        for k in probe:
+            mixed = _answers(s, ids[k])
+            _log("alone vs mix", k, lengths[k])
+            assert np.array_equal(mixed, alone[k]), "job %d: mix differs from alone" % k
+            assert np.array_equal(mixed, oracle.apsep(host[k])), "job %d vs oracle" % k
+        for jid in ids:
+            s.release(jid)
+
+
 def test_budget_admission_and_wraparound(torch_mod):
     """A pool that holds a few jobs at a time: submissions are refused until
     finished jobs are released; slots wrap around the ring."""
