@@ -11,6 +11,12 @@
       scaled by what they read for a plain device copy of N int32 in the
       same run (the copy moves exactly 4N bytes each way), so the result is
       in true bytes.
+
+  python -m apsep_amd.rocprof_summary counters DB [DB ...] --kernel NAME
+          [--skip K] [--command CMD]
+      mean per dispatch of every counter recorded in the given ``--pmc``
+      databases for the kernels whose name contains NAME (the first K
+      dispatches skipped), and per wave when SQ_WAVES is among them.
 """
 from __future__ import annotations
 
@@ -46,6 +52,11 @@ def pmc_per_dispatch(db: str, counter: str):
     return out
 
 
+def counter_names(db: str) -> list[str]:
+    con = sqlite3.connect(db)
+    return [r[0] for r in con.execute("select distinct counter_name from pmc_events")]
+
+
 def _copy_kernel(per: dict[str, list[float]]):
     """The calibration copy: the copy kernel with the most dispatches."""
     cands = [(len(v), k) for k, v in per.items() if "copy" in k.lower()]
@@ -65,7 +76,29 @@ def main(argv=None):
     t.add_argument("--algo", default="wstl")
     t.add_argument("--json", default=None)
     t.add_argument("--command", default="")
+    c = sub.add_parser("counters")
+    c.add_argument("dbs", nargs="+")
+    c.add_argument("--kernel", required=True)
+    c.add_argument("--skip", type=int, default=2)
+    c.add_argument("--command", default="")
     a = ap.parse_args(argv)
+
+    if a.cmd == "counters":
+        vals = {}
+        for db in a.dbs:
+            for cn in counter_names(db):
+                for name, v in pmc_per_dispatch(db, cn).items():
+                    if a.kernel in name:
+                        v = v[a.skip:] if len(v) > a.skip else v
+                        vals[cn] = statistics.mean(v)
+        if a.command:
+            print("command: %s" % a.command)
+        waves = vals.get("SQ_WAVES")
+        print("%-24s %16s %12s" % ("counter", "per dispatch", "per wave"))
+        for cn in sorted(vals):
+            pw = "%12.1f" % (vals[cn] / waves) if waves else ""
+            print("%-24s %16.0f %s" % (cn, vals[cn], pw))
+        return 0
 
     if a.cmd == "kernels":
         if a.command:
