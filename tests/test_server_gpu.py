@@ -96,6 +96,25 @@ def test_result_alone_equals_result_in_mix(torch_mod):
             s.release(jid)
 
 
+def test_inclusive_predicate_server(torch_mod):
+    """A server created for "<=": every job's answers are the oracle's
+    "<=" answers (ties answer to the nearest equal key)."""
+    import apsep_amd as A
+    torch = torch_mod
+    rng = np.random.default_rng(9)
+    lengths = [int(x) for x in rng.integers(1, 120000, size=20)] + [1, 2, 4096, 4097]
+    host, dev = _jobs(torch, lengths, 9)
+    torch.cuda.synchronize()
+    with A.Server(1 << 28, 1 << 22, 16, incl=True) as s:
+        ids = [s.submit(dev[k].data_ptr(), n) for k, n in enumerate(lengths)]
+        assert all(j is not None for j in ids)
+        s.drain()
+        for k, jid in enumerate(ids):
+            got = _answers(s, jid)
+            assert np.array_equal(got, oracle.apsep(host[k], True)), "job %d n=%d" % (k, lengths[k])
+            s.release(jid)
+
+
 def test_budget_admission_and_wraparound(torch_mod):
     """A pool that holds a few jobs at a time: submissions are refused until
     finished jobs are released; slots wrap around the ring."""
