@@ -70,7 +70,8 @@ EDGE_LENGTHS = [0, 1, 2, 15, 16, 17, 0, 63, 64, 65, 4095, 4096, 4097, 1, 0, 8191
 
 @pytest.mark.parametrize("lanes", [256, 512])
 @pytest.mark.parametrize("incl", [False, True])
-@pytest.mark.parametrize("kind", ["random", "descending", "ascending", "dup2", "mixed"])
+@pytest.mark.parametrize("kind", ["random", "descending", "ascending", "dup2", "mixed",
+                                  "intmax", "alleq"])
 def test_edge_lengths(torch_mod, kind, incl, lanes):
     import apsep_amd as A
     lengths = EDGE_LENGTHS
