@@ -155,6 +155,14 @@ def test_tile_and_level_boundaries(torch_mod, algo):
                 check(solve_on(s, torch, a, False), a, False, "%s n=%d %s" % (algo, n, kind))
             a = oracle.fill("dup16", n, seed=n + 1)
             check(solve_on(s, torch, a, True), a, True, "%s n=%d dup16 incl" % (algo, n))
+            # equal keys across tile edges: a tile's first element answers to the
+            # key before the tile under "<=" only; INT_MAX is a legal key there
+            a = oracle.fill("alleq", n, seed=n)
+            for incl in (False, True):
+                check(solve_on(s, torch, a, incl), a, incl,
+                      "%s n=%d alleq incl=%d" % (algo, n, incl))
+            a = oracle.fill("intmax", n, seed=n + 2)
+            check(solve_on(s, torch, a, True), a, True, "%s n=%d intmax incl" % (algo, n))
 
 
 @pytest.mark.parametrize("algo", ALGOS)
