@@ -148,6 +148,65 @@ def scratch_bytes(algo, n_cap: int, incl: bool = False) -> int:
     return b.value
 
 
+def _device_index(t) -> int:
+    return t.device.index if t.device.index is not None else 0
+
+
+def check_single_args(x, out, n: int, n_cap: int, device: int):
+    """Validate a solve of ``n`` keys of ``x`` into ``out`` before any launch:
+    int32 dtypes, 0 <= n <= both sizes and the capacity, contiguous device
+    tensors on the scratch's device.  Raises TypeError / ValueError; touches
+    no device."""
+    import torch
+    if x.dtype != torch.int32 or out.dtype != torch.int32:
+        raise TypeError("apsep: int32 tensors required")
+    if n < 0:
+        raise ValueError("apsep: n = %d is negative" % n)
+    if n > x.numel() or n > out.numel():
+        raise ValueError("apsep: n = %d exceeds the input (%d) or output (%d) size"
+                         % (n, x.numel(), out.numel()))
+    if n > n_cap:
+        raise ValueError("apsep: n = %d exceeds the scratch capacity %d" % (n, n_cap))
+    if not (x.is_contiguous() and out.is_contiguous()):
+        raise ValueError("apsep: contiguous tensors required")
+    if not (x.is_cuda and out.is_cuda):
+        raise ValueError("apsep: device tensors required")
+    if _device_index(x) != device or _device_index(out) != device:
+        raise ValueError("apsep: tensors on device %d/%d, scratch on device %d"
+                         % (_device_index(x), _device_index(out), device))
+
+
+def check_segmented_args(x, offsets, out, total: int, total_cap: int, nseg_cap: int,
+                         device: int):
+    """Validate a segmented solve before any launch: int32 keys and answers,
+    a 1-D int64 offsets tensor of nseg + 1 entries with nseg <= nseg_cap,
+    0 <= total <= both sizes and the capacity, contiguous device tensors on
+    the plan's device.  Raises TypeError / ValueError; touches no device."""
+    import torch
+    if x.dtype != torch.int32 or out.dtype != torch.int32 or offsets.dtype != torch.int64:
+        raise TypeError("apsep: int32 keys/answers and int64 offsets required")
+    if offsets.dim() != 1 or offsets.numel() < 1:
+        raise ValueError("apsep: offsets must be 1-D with nseg + 1 >= 1 entries")
+    nseg = offsets.numel() - 1
+    if nseg > nseg_cap:
+        raise ValueError("apsep: %d segments exceed the plan's %d" % (nseg, nseg_cap))
+    if total < 0:
+        raise ValueError("apsep: total = %d is negative" % total)
+    if total > x.numel() or total > out.numel():
+        raise ValueError("apsep: total = %d exceeds the keys (%d) or answers (%d) size"
+                         % (total, x.numel(), out.numel()))
+    if total > total_cap:
+        raise ValueError("apsep: total = %d exceeds the plan capacity %d" % (total, total_cap))
+    if not (x.is_contiguous() and out.is_contiguous() and offsets.is_contiguous()):
+        raise ValueError("apsep: contiguous tensors required")
+    if not (x.is_cuda and out.is_cuda and offsets.is_cuda):
+        raise ValueError("apsep: device tensors required")
+    devs = {_device_index(x), _device_index(out), _device_index(offsets)}
+    if devs != {device}:
+        raise ValueError("apsep: tensors on devices %s, plan on device %d"
+                         % (sorted(devs), device))
+
+
 class Scratch:
     """Device scratch for solves of up to ``n_cap`` elements."""
 
@@ -161,6 +220,7 @@ class Scratch:
         self.info = ScratchInfo()
         _check(lib().apsep_scratch_query(self._h, ctypes.byref(self.info)))
         self.algo = self.info.algo
+        self.device = self.info.device
 
     def run_ptr(self, d_in: int, d_out: int, n: int, stream: int = 0,
                 incl: bool | None = None):
@@ -177,10 +237,7 @@ class Scratch:
         import torch
         if n is None:
             n = x.numel()
-        if x.dtype != torch.int32 or out.dtype != torch.int32:
-            raise TypeError("apsep: int32 tensors required")
-        if not (x.is_cuda and out.is_cuda and x.is_contiguous() and out.is_contiguous()):
-            raise ValueError("apsep: contiguous device tensors required")
+        check_single_args(x, out, n, self.n_cap, self.device)
         if stream is None:
             stream = torch.cuda.current_stream(x.device)
         self.run_ptr(x.data_ptr(), out.data_ptr(), n, stream.cuda_stream, incl)
@@ -201,6 +258,7 @@ class Scratch:
         import torch
         if n is None:
             n = x.numel()
+        check_single_args(x, out, n, self.n_cap, self.device)
         if stream is None:
             stream = torch.cuda.current_stream(x.device)
         t = KernelTimes()
@@ -236,9 +294,13 @@ class BatchPlan:
                  graph: bool = False):
         self._h = ctypes.c_void_p(None)
         self.incl = bool(incl)
+        self.total_cap = total_cap
+        self.nseg_cap = nseg_cap
         _check(lib().apsep_batch_plan_create(total_cap, nseg_cap, int(incl),
                                              ctypes.byref(self._h)),
                "apsep_batch_plan_create")
+        import torch
+        self.device = torch.cuda.current_device()  # the plan binds to it
         if graph:
             self.use_graph(True)
 
@@ -265,6 +327,8 @@ class BatchPlan:
         import torch
         if total is None:
             total = x.numel()
+        check_segmented_args(x, offsets, out, total, self.total_cap, self.nseg_cap,
+                             self.device)
         if stream is None:
             stream = torch.cuda.current_stream(x.device)
         e, g = ctypes.c_float(0), ctypes.c_float(0)
@@ -283,12 +347,10 @@ class BatchPlan:
         """Solve every segment of ``x`` (int32) given ``offsets`` (int64, nseg + 1,
         on the device); answers relative to each segment start go to ``out``."""
         import torch
-        if x.dtype != torch.int32 or out.dtype != torch.int32 or offsets.dtype != torch.int64:
-            raise TypeError("apsep: int32 keys/answers and int64 offsets required")
-        if not (x.is_cuda and out.is_cuda and offsets.is_cuda):
-            raise ValueError("apsep: device tensors required")
         if total is None:
             total = x.numel()
+        check_segmented_args(x, offsets, out, total, self.total_cap, self.nseg_cap,
+                             self.device)
         if stream is None:
             stream = torch.cuda.current_stream(x.device)
         self.run_ptr(x.data_ptr(), offsets.data_ptr(), out.data_ptr(),
@@ -359,19 +421,26 @@ class Server:
         return ji
 
     def answers(self, job_id: int, out=None):
-        """Copy a done job's answers into ``out`` (int32 device tensor of n;
-        allocated when None) on torch's current stream."""
+        """Copy a done job's answers into ``out`` (int32 device tensor of at
+        least n; allocated when None) on torch's current stream, and wait for
+        the copy: the job's slot may be released (and reused by the server's
+        next batch) as soon as this returns."""
         import torch
         info = self.info(job_id)
         if info.state != JOB_DONE:
             raise ApsepError(ERR_INVALID_ARG, "job %d is not done" % job_id)
         if out is None:
             out = torch.empty(info.n, dtype=torch.int32, device="cuda")
+        if out.dtype != torch.int32 or not out.is_cuda or not out.is_contiguous() \
+                or out.numel() < info.n:
+            raise ValueError("apsep: answers need a contiguous int32 device tensor of "
+                             ">= %d elements" % info.n)
         if info.n:
-            st = torch.cuda.current_stream().cuda_stream
+            cs = torch.cuda.current_stream()
             if _hip().hipMemcpyAsync(out.data_ptr(), info.d_answers, info.n * 4,
-                                     3, st) != 0:  # hipMemcpyDeviceToDevice
+                                     3, cs.cuda_stream) != 0:  # hipMemcpyDeviceToDevice
                 raise ApsepError(ERR_LAUNCH, "hipMemcpyAsync")
+            cs.synchronize()
         return out
 
     def release(self, job_id: int):

@@ -111,7 +111,7 @@ struct Levels {
 // the tile pass folds every tile min into its group's key with a device
 // atomic, and atomics on keys sharing a line serialise (packed keys cost the
 // tile pass ~4 us at N = 2^25 on MI355X).  Upper levels are packed.
-__host__ __device__ __forceinline__ size_t key_slot(int lvl, int idx) {
+__host__ __device__ constexpr size_t key_slot(int lvl, int idx) {
     return (size_t)idx << (lvl == 1 ? K1_LOG : 0);
 }
 __device__ __forceinline__ u64& key_at(const Levels& L, int lvl, int idx) {
@@ -1365,9 +1365,13 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
 // Grid barrier for persistent cooperative launches.  Arrivals are spread
 // over 64 sub-counters (one 128-byte line each); the last arriver of a
 // sub-group arrives at the root; the last root arriver bumps the generation
-// word (its own line).  The spin is bounded: a grid that is not fully
-// co-resident (or any other stall) raises `err` and lets every workgroup
-// leave instead of hanging the device.
+// word (its own line).  The counters only grow inside a launch (barrier j
+// of a launch is complete for a sub-group at (j + 1) x its size), and a
+// one-workgroup kernel (grid_barrier_reset) zeroes them before every launch,
+// so a launch cut short by a timeout leaves nothing behind for the next.
+// The spin is bounded: a grid that is not fully co-resident (or any other
+// stall) raises `err` and lets every workgroup leave instead of hanging the
+// device; `err` stays set until the host reads it (apsep_scratch_check).
 // ---------------------------------------------------------------------------
 constexpr int BAR_SUB = 64;
 constexpr int BAR_LINE = 32;  // uints per 128-byte line
@@ -1383,6 +1387,13 @@ struct GridBarrier {
 
 constexpr unsigned BARRIER_SPIN_LIMIT = 1u << 22;  // >= 1 s at >= 250 ns per poll
 
+// Launched (one workgroup of BAR_SUB lanes) before each persistent launch.
+__device__ __forceinline__ void grid_barrier_reset(GridBarrier* b) {
+    const int i = (int)threadIdx.x;
+    if (i < BAR_SUB) b->sub[i * BAR_LINE] = 0u;
+    if (i == 0) b->root = 0u;
+}
+
 // Generation a launch starts from.  The counter persists across launches;
 // it cannot advance before every workgroup has arrived at the first
 // barrier, and each workgroup reads it before arriving, so all workgroups
@@ -1391,9 +1402,10 @@ __device__ __forceinline__ unsigned grid_barrier_start(GridBarrier* b) {
     return __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Returns false (uniformly per workgroup) if the barrier timed out.
+// Barrier number `j` (0, 1, ...) of the launch.  Returns false (uniformly
+// per workgroup) if the barrier timed out.
 __device__ __forceinline__ bool grid_barrier(GridBarrier* b, unsigned nblocks,
-                                             unsigned& gen) {
+                                             unsigned& gen, unsigned& j) {
     __shared__ int s_ok;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1403,14 +1415,9 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier* b, unsigned nblocks,
         const unsigned nsub = nblocks < (unsigned)BAR_SUB ? nblocks : (unsigned)BAR_SUB;
         const unsigned gsz = nblocks / BAR_SUB + (g < nblocks % BAR_SUB ? 1u : 0u);
         bool release = false;
-        if (atomicAdd(&b->sub[g * BAR_LINE], 1u) == gsz - 1) {
-            __hip_atomic_store(&b->sub[g * BAR_LINE], 0u, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            if (atomicAdd(&b->root, 1u) == nsub - 1) {
-                __hip_atomic_store(&b->root, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                release = true;
-            }
-        }
+        if (atomicAdd(&b->sub[g * BAR_LINE], 1u) == (j + 1) * gsz - 1 &&
+            atomicAdd(&b->root, 1u) == (j + 1) * nsub - 1)
+            release = true;
         if (release) {
             __hip_atomic_store(&b->gen, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
@@ -1429,6 +1436,7 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier* b, unsigned nblocks,
     }
     __syncthreads();
     gen++;
+    j++;
     return s_ok != 0;
 }
 

@@ -33,6 +33,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     tile_step<INCL>(in, out, n, (int)blockIdx.x, sm, to, L);
 }
 
+// Level-1 keys emptied (above every key of every run).  Launched before the
+// tile pass of a run recorded into a graph: such a run uses the fixed epoch
+// CAPTURE_EPOCH, so that each replay starts from clean keys instead of
+// folding its tile mins into the previous replay's.
+__global__ __launch_bounds__(256) void k_reset_level1(u64* __restrict__ key1, int nkey1) {
+    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < nkey1;
+         i += (int)(gridDim.x * blockDim.x))
+        key1[key_slot(1, i)] = ~0ull;
+}
+
+// Before every SPT launch: the grid barrier's arrival counters zeroed.
+__global__ __launch_bounds__(BAR_SUB) void k_spt_begin(GridBarrier* bar) {
+    grid_barrier_reset(bar);
+}
+
 // WSTL pass 2 (only when level 1 exceeds UPPER_MIN entries): levels >= 2.
 __global__ __launch_bounds__(256) void k_upper_levels(Levels L) { build_upper_levels(L); }
 
@@ -64,15 +79,15 @@ void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
            TileOut to, Levels L, GridBarrier* bar) {
     __shared__ TileSmem sm;
     __shared__ int lds[NWAVE][FIX_LDS_INTS];
-    unsigned gen = grid_barrier_start(bar);
+    unsigned gen = grid_barrier_start(bar), j = 0;
     const int nt = L.nt;
     for (int t = (int)blockIdx.x; t < nt; t += (int)gridDim.x)
         tile_step<INCL>(in, out, n, t, sm, to, L);
     if (nt <= 1) return;  // uniform across the grid
-    if (!grid_barrier(bar, gridDim.x, gen)) return;
+    if (!grid_barrier(bar, gridDim.x, gen, j)) return;
     if (L.upper) {
         if (blockIdx.x == 0) build_upper_levels(L);
-        if (!grid_barrier(bar, gridDim.x, gen)) return;
+        if (!grid_barrier(bar, gridDim.x, gen, j)) return;
     }
     const int gw = (int)(blockIdx.x * NWAVE + (threadIdx.x >> 6));
     const int nw = (int)(gridDim.x * NWAVE);
@@ -84,50 +99,81 @@ void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
 // Host side
 // ---------------------------------------------------------------------------
 
-static inline int64_t round4(int64_t x) { return (x + 3) & ~(int64_t)3; }
+// Run epochs.  Eager runs take 2, 3, ... from the scratch; a run recorded
+// into a graph takes CAPTURE_EPOCH and empties level 1 first (its launch
+// arguments are replayed unchanged).  A later eager run's keys compare below
+// a captured run's, so the two kinds mix on one scratch in any order.
+constexpr unsigned CAPTURE_EPOCH = 1u;
+constexpr unsigned FIRST_EAGER_EPOCH = 2u;
 
-// Level sizes for nt tiles; returns nlev.
-int level_sizes(int nt, int* sizes) {
-    int nl = 0, m = nt;
+constexpr int64_t round4(int64_t x) { return (x + 3) & ~(int64_t)3; }
+
+struct LevelSizes {
+    int nl;
+    int sz[MAX_LEVELS];
+};
+
+// Level sizes for nt tiles: entries per level, the top one <= FAN.
+constexpr LevelSizes level_sizes_of(int nt) {
+    LevelSizes r{0, {0, 0, 0, 0, 0}};
+    int m = nt;
     while (true) {
-        sizes[nl++] = m;
+        r.sz[r.nl++] = m;
         if (m <= FAN) break;
         m = (m + FAN - 1) / FAN;
     }
-    return nl;
+    return r;
+}
+
+// Level sizes for nt tiles; returns nlev.
+int level_sizes(int nt, int* sizes) {
+    const LevelSizes r = level_sizes_of(nt);
+    for (int l = 0; l < r.nl; l++) sizes[l] = r.sz[l];
+    return r.nl;
 }
 
 // Chunk-min slots: whole tiles of the scratch's own shape and of the
 // largest shape (this file's tile): a batch plan's scratch, sized for the
 // smaller segmented tile, also serves batches run on the larger one, whose
 // last partial tile writes all of its chunk mins.
-static int64_t chunk_slots(int64_t n_cap, int tile) {
+constexpr int64_t chunk_slots(int64_t n_cap, int tile) {
     const int64_t own = (n_cap + tile - 1) / tile * (tile / CHUNK);
     const int64_t big = (n_cap + TILE - 1) / TILE * CPT;
     return own > big ? own : big;
 }
 
-int64_t single_scratch_bytes(int64_t n_cap, apsep_algo algo, int tile) {
+constexpr int64_t ALIGN_SLACK = 256 * 12;
+
+// Bytes of the arrays every solver uses (tile mins, keyed levels, chunk
+// mins, open lists, spines, grid barrier), without the alignment slack.
+constexpr int64_t common_scratch_bytes(int64_t n_cap, int tile) {
     const int nt = (int)((n_cap + tile - 1) / tile);
     const int64_t ncm = chunk_slots(n_cap, tile);
-    int sz[MAX_LEVELS];
-    const int nl = level_sizes(nt < 1 ? 1 : nt, sz);
-    int64_t b = (round4(sz[0]) + 4) * 4;             // tile mins
-    for (int l = 1; l < nl; l++) b += ((int64_t)key_slot(l, round4(sz[l]) + 4)) * 8;  // keyed levels
+    const LevelSizes ls = level_sizes_of(nt < 1 ? 1 : nt);
+    int64_t b = (round4(ls.sz[0]) + 4) * 4;          // tile mins
+    for (int l = 1; l < ls.nl; l++) b += ((int64_t)key_slot(l, (int)round4(ls.sz[l]) + 4)) * 8;
     b += (round4(ncm) + 4) * 4;                       // cmin
     b += (round4(nt) + 4) * 4;                        // qcount
     b += ((int64_t)nt * QS + 2) * 8;                  // qlist
     b += (round4(nt) + 4) * 4;                        // scount
     b += ((int64_t)nt * RS + 2) * 8;                  // spine
     b += (int64_t)sizeof(GridBarrier);                // grid barrier
+    return b;
+}
+
+int64_t single_scratch_bytes(int64_t n_cap, apsep_algo algo, int tile) {
+    int64_t b = common_scratch_bytes(n_cap, tile);
     if (algo == APSEP_ALGO_LOOKBACK) {
+        const int nt = (int)((n_cap + tile - 1) / tile);
+        const int64_t ncm = chunk_slots(n_cap, tile);
+        const LevelSizes ls = level_sizes_of(nt < 1 ? 1 : nt);
         b += (int64_t)sizeof(LbCtl);
         b += 2 * (round4(nt) + 4) * 8;                // tile mins, inclusive prefixes
         b += (round4(ncm) + 4) * 8;                   // chunk mins
-        for (int l = 1; l < nl; l++) b += (round4(sz[l]) + 4) * 8;  // group mins
+        for (int l = 1; l < ls.nl; l++) b += (round4(ls.sz[l]) + 4) * 8;  // group mins
         b += 256 * (4 + LB_LEVELS);
     }
-    return b + 256 * 12;                              // alignment slack
+    return b + ALIGN_SLACK;                           // alignment slack
 }
 
 static void* carve(char*& p, int64_t bytes) {
@@ -136,23 +182,16 @@ static void* carve(char*& p, int64_t bytes) {
     return (void*)a;
 }
 
-apsep_status single_scratch_init(apsep_scratch* s) {
+// Point the scratch's arrays into s->base (s->bytes long), sized for
+// s->n_cap keys in tiles of s->tile; false if they do not fit.
+static bool carve_arrays(apsep_scratch* s) {
     const int nt = (int)((s->n_cap + s->tile - 1) / s->tile);
     const int64_t ncm = chunk_slots(s->n_cap, s->tile);
-    s->nt_cap = nt;
-    if (nt == 0) return APSEP_OK;
     int sz[MAX_LEVELS];
     const int nl = level_sizes(nt, sz);
-    s->bytes = single_scratch_bytes(s->n_cap, s->algo, s->tile);
-    if (hipMalloc(&s->base, (size_t)s->bytes) != hipSuccess) {
-        s->base = nullptr;
-        (void)hipGetLastError();
-        return APSEP_ERR_ALLOC;
-    }
     char* p = (char*)s->base;
     s->mn0 = (int*)carve(p, (round4(sz[0]) + 4) * 4);
     for (int l = 1; l < nl; l++) s->key[l] = (u64*)carve(p, (int64_t)key_slot(l, round4(sz[l]) + 4) * 8);
-    s->epoch = 1;
     s->cmin = (int*)carve(p, (round4(ncm) + 4) * 4);
     s->qcount = (int*)carve(p, (round4(nt) + 4) * 4);
     s->qlist = (int2*)carve(p, ((int64_t)nt * QS + 2) * 8);
@@ -166,7 +205,23 @@ apsep_status single_scratch_init(apsep_scratch* s) {
         s->lb_CM = (u64*)carve(p, (round4(ncm) + 4) * 8);
         for (int l = 1; l < nl; l++) s->lb_G[l] = (u64*)carve(p, (round4(sz[l]) + 4) * 8);
     }
-    if ((char*)p > (char*)s->base + s->bytes) return APSEP_ERR_ALLOC;  // sizing bug guard
+    return (char*)p <= (char*)s->base + s->bytes;
+}
+
+apsep_status single_scratch_init(apsep_scratch* s) {
+    const int nt = (int)((s->n_cap + s->tile - 1) / s->tile);
+    s->nt_cap = nt;
+    if (nt == 0) return APSEP_OK;
+    int sz[MAX_LEVELS];
+    const int nl = level_sizes(nt, sz);
+    s->bytes = single_scratch_bytes(s->n_cap, s->algo, s->tile);
+    if (hipMalloc(&s->base, (size_t)s->bytes) != hipSuccess) {
+        s->base = nullptr;
+        (void)hipGetLastError();
+        return APSEP_ERR_ALLOC;
+    }
+    if (!carve_arrays(s)) return APSEP_ERR_ALLOC;  // sizing bug guard
+    s->epoch = FIRST_EAGER_EPOCH;
     if (hipMemset(s->base, 0, (size_t)s->bytes) != hipSuccess) {
         (void)hipGetLastError();
         return APSEP_ERR_LAUNCH;
@@ -202,8 +257,21 @@ apsep_status single_scratch_init(apsep_scratch* s) {
     return APSEP_OK;
 }
 
-// Levels for a run over nt tiles; advances the scratch's run epoch.
-static apsep_status fill_levels(apsep_scratch* s, int nt, Levels& L, hipStream_t st) {
+// 0: the stream is not capturing; 1: it is; -1: it cannot be told (an
+// invalidated capture, or the legacy stream under another stream's capture).
+static int capture_state(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    return cs == hipStreamCaptureStatusNone ? 0 : cs == hipStreamCaptureStatusActive ? 1 : -1;
+}
+
+// Levels for a run over nt tiles.  Eager runs advance the scratch's run
+// epoch; a captured run (capturing = 1) launches the level-1 reset.
+static apsep_status fill_levels(apsep_scratch* s, int nt, Levels& L, hipStream_t st,
+                                int capturing = 0) {
     int sz[MAX_LEVELS];
     L.nt = nt;
     L.nlev = level_sizes(nt, sz);
@@ -213,6 +281,15 @@ static apsep_status fill_levels(apsep_scratch* s, int nt, Levels& L, hipStream_t
         L.n[l] = l < L.nlev ? sz[l] : 0;
         L.key[l] = (l >= 1 && l < L.nlev) ? s->key[l] : nullptr;
     }
+    if (capturing) {
+        if (L.nlev > 1) {
+            const int blocks = (sz[1] + 255) / 256;
+            hipLaunchKernelGGL(k_reset_level1, dim3(blocks), dim3(256), 0, st, s->key[1], sz[1]);
+            if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
+        }
+        L.epoch = CAPTURE_EPOCH;
+        return APSEP_OK;
+    }
     if (s->epoch >= 0xFFFFFFF0u) {
         // epoch wrap (after ~4e9 runs): clear every key so none can alias
         int szc[MAX_LEVELS];
@@ -220,7 +297,7 @@ static apsep_status fill_levels(apsep_scratch* s, int nt, Levels& L, hipStream_t
         for (int l = 1; l < nlc; l++)
             if (hipMemsetAsync(s->key[l], 0xFF, key_slot(l, szc[l]) * 8, st) != hipSuccess)
                 return APSEP_ERR_LAUNCH;
-        s->epoch = 1;
+        s->epoch = FIRST_EAGER_EPOCH;
     }
     L.epoch = s->epoch++;
     return APSEP_OK;
@@ -230,8 +307,10 @@ template <bool INCL>
 static apsep_status run_wstl(apsep_scratch* s, const int* in, int* out, int n,
                              hipStream_t st) {
     const int nt = (int)(((long long)n + TILE - 1) / TILE);  // (n + TILE - 1 overflows near 2^31)
+    const int cap = capture_state(st);
+    if (cap < 0) return APSEP_ERR_UNSUPPORTED;
     Levels L;
-    if (fill_levels(s, nt, L, st) != APSEP_OK) return APSEP_ERR_LAUNCH;
+    if (fill_levels(s, nt, L, st, cap) != APSEP_OK) return APSEP_ERR_LAUNCH;
     TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
     hipLaunchKernelGGL(k_tile_pass<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L);
     if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
@@ -249,6 +328,8 @@ static apsep_status run_wstl(apsep_scratch* s, const int* in, int* out, int n,
 template <bool INCL>
 static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
                             hipStream_t st) {
+    // a cooperative launch is not recorded into graphs
+    if (capture_state(st) != 0) return APSEP_ERR_UNSUPPORTED;
     const int nt = (int)(((long long)n + TILE - 1) / TILE);  // (n + TILE - 1 overflows near 2^31)
     Levels L;
     if (fill_levels(s, nt, L, st) != APSEP_OK) return APSEP_ERR_LAUNCH;
@@ -256,6 +337,8 @@ static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
     int grid = s->coop_grid < nt ? s->coop_grid : nt;
     if (grid < 1) grid = 1;
     GridBarrier* bar = s->bar;
+    hipLaunchKernelGGL(k_spt_begin, dim3(1), dim3(BAR_SUB), 0, st, bar);
+    if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
     void* args[] = {(void*)&in, (void*)&out, (void*)&n, (void*)&to, (void*)&L, (void*)&bar};
     hipError_t e = hipLaunchCooperativeKernel((const void*)k_spt<INCL>, dim3(grid), dim3(NT),
                                               args, 0, st);
@@ -294,6 +377,7 @@ static apsep_status time_wstl(apsep_scratch* s, const int* in, int* out, int n, 
 
 apsep_status single_time_kernels(apsep_scratch* s, const int32_t* in, int32_t* out, int n,
                                  int incl, int reps, hipStream_t st, double ms[3]) {
+    if (capture_state(st) != 0) return APSEP_ERR_UNSUPPORTED;  // it synchronises
     hipEvent_t ev[4];
     int made = 0;
     apsep_status r = APSEP_OK;
@@ -363,9 +447,9 @@ apsep_status single_error(apsep_scratch* s, int* code) {
         return APSEP_ERR_LAUNCH;
     }
     if (err) {
-        // a timed-out barrier leaves its counts behind: reset the whole state
-        if (hipMemset(s->bar, 0, sizeof(GridBarrier)) != hipSuccess ||
-            hipDeviceSynchronize() != hipSuccess) {
+        // cleared once read (the counters are zeroed before every launch)
+        const int zero = 0;
+        if (hipMemcpy(&s->bar->err, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
             (void)hipGetLastError();
             return APSEP_ERR_LAUNCH;
         }
