@@ -76,6 +76,7 @@ def lib():
                                           i32p, ctypes.c_int32]
         L.oracle_stress_trial.restype = ctypes.c_int32
         L.oracle_glibc_rand_fill.argtypes = [ctypes.c_uint32, ctypes.c_int64, i32p]
+        L.oracle_glibc_intmax_fill.argtypes = [ctypes.c_uint32, ctypes.c_int64, i32p]
         L.oracle_fill.argtypes = [ctypes.c_int32, ctypes.c_uint64,
                                   ctypes.c_int64, ctypes.c_int64,
                                   ctypes.c_int64, i32p]
@@ -175,6 +176,15 @@ def glibc_rand(seed: int, n: int) -> np.ndarray:
     """``srand(seed)`` then ``n`` x ``rand()`` (glibc), restated."""
     out = np.empty(n, dtype=np.int32)
     lib().oracle_glibc_rand_fill(seed & 0xFFFFFFFF, n, _p32(out))
+    return out
+
+
+def glibc_intmax(seed: int, n: int) -> np.ndarray:
+    """``srand(seed)``; ``h[i] = (rand() % 4 == 0) ? INT_MAX : rand()``
+    (src/bench_spt_le.cu:122-125), restated."""
+    out = np.empty(n, dtype=np.int32)
+    if lib().oracle_glibc_intmax_fill(seed & 0xFFFFFFFF, n, _p32(out)):
+        raise MemoryError("oracle_glibc_intmax_fill")
     return out
 
 

@@ -29,6 +29,8 @@
  *   oracle_glibc_rand_fill  glibc rand() after srand(seed) (TYPE_3 additive
  *                           feedback generator, r[i] = r[i-3] + r[i-31]),
  *                           used by src/bench_spt_le.cu:93-125
+ *   oracle_glibc_intmax_fill  that generator's INT_MAX-heavy input
+ *                           (src/bench_spt_le.cu:122-125)
  *   oracle_fill_*           workload generators for the benchmark classes
  *
  * Parity status: every function above is pinned by tests/test_oracle.py
@@ -292,6 +294,30 @@ void oracle_glibc_rand_fill(uint32_t seed, int64_t n, int32_t* out)
         ring[k % 34] = x;
         out[i] = (int32_t)(x >> 1);
     }
+}
+
+/* The INT_MAX-heavy input of src/bench_spt_le.cu:122-125: srand(seed), then
+ * per element h[i] = (rand() % 4 == 0) ? INT_MAX : rand() -- the second draw
+ * is made only when the first is not a multiple of 4, so element i consumes
+ * one or two outputs of the same rand() stream.  Returns 0, 1 on OOM. */
+int oracle_glibc_intmax_fill(uint32_t seed, int64_t n, int32_t* out)
+{
+    if (n <= 0) return 0;
+    int32_t* r = (int32_t*)malloc((size_t)(2 * n + 1) * sizeof(int32_t));
+    if (!r) return 1;
+    oracle_glibc_rand_fill(seed, 2 * n + 1, r);
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (r[k] % 4 == 0) {
+            out[i] = 2147483647;
+            k += 1;
+        } else {
+            out[i] = r[k + 1];
+            k += 2;
+        }
+    }
+    free(r);
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */

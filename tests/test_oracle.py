@@ -202,3 +202,19 @@ def test_glibc_rand_matches_libc():
         assert got[:8] == g["glibc_rand"][str(seed)]
     # the classic first outputs of srand(1)
     assert oracle.glibc_rand(1, 2).tolist() == [1804289383, 846930886]
+
+
+def test_glibc_intmax_matches_libc():
+    """The INT_MAX-heavy input of src/bench_spt_le.cu:122-125, drawn here
+    straight from libc's own rand(): the second draw happens only when the
+    first is not a multiple of 4."""
+    libc = ctypes.CDLL("libc.so.6")
+    n = 20000
+    libc.srand(5)
+    want = []
+    for _ in range(n):
+        want.append(2147483647 if libc.rand() % 4 == 0 else libc.rand())
+    got = oracle.glibc_intmax(5, n).tolist()
+    assert got == want
+    frac = sum(v == 2147483647 for v in got) / n
+    assert 0.22 < frac < 0.28

@@ -75,7 +75,9 @@ def test_known_answers(torch_mod, algo):
 
 @pytest.mark.parametrize("algo,seed,trials,maxn", [
     ("wstl", 42, 500, 4096), ("spt", 43, 500, 4096),
-    ("wstl", 100, 200, 131072), ("spt", 101, 200, 131072)])
+    ("wstl", 100, 200, 131072), ("spt", 101, 200, 131072),
+    ("lookback", 42, 500, 4096), ("lookback", 43, 500, 4096),
+    ("lookback", 100, 200, 131072), ("lookback", 101, 200, 131072)])
 def test_reference_stress(torch_mod, algo, seed, trials, maxn):
     import apsep_amd as A
     torch = torch_mod
@@ -99,32 +101,19 @@ LE_KINDS = {
     "all-equal": lambda N: np.full(N, 42, dtype=np.int32),
     "descending": lambda N: (N - np.arange(N)).astype(np.int32),
     "ascending": lambda N: np.arange(N, dtype=np.int32),
+    # srand(5); h[i] = (rand() % 4 == 0) ? INT_MAX : rand()
+    # (src/bench_spt_le.cu:122-125)
+    "intmax": lambda N: oracle.glibc_intmax(5, N),
 }
 
 
-def _intmax(N):
-    # srand(5); h[i] = (rand() % 4 == 0) ? INT_MAX : rand()  (two draws/elt
-    # when the first is not a multiple of 4, src/bench_spt_le.cu:122-125)
-    r = oracle.glibc_rand(5, 2 * N + 16)
-    out = np.empty(N, dtype=np.int32)
-    k = 0
-    for i in range(N):
-        if r[k] % 4 == 0:
-            out[i] = 2147483647
-            k += 1
-        else:
-            out[i] = r[k + 1]
-            k += 2
-    return out
-
-
 @pytest.mark.parametrize("algo", ALGOS)
-@pytest.mark.parametrize("kind", list(LE_KINDS) + ["intmax"])
+@pytest.mark.parametrize("kind", list(LE_KINDS))
 def test_le_matrix(torch_mod, algo, kind):
     import apsep_amd as A
     torch = torch_mod
-    N = 1 << 25 if kind != "intmax" else 1 << 21
-    h = LE_KINDS[kind](N) if kind != "intmax" else _intmax(N)
+    N = 1 << 25
+    h = LE_KINDS[kind](N)
     h = np.ascontiguousarray(h, dtype=np.int32)
     x = torch.from_numpy(h).cuda()
     out = torch.empty(N, dtype=torch.int32, device="cuda")
