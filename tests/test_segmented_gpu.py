@@ -232,3 +232,67 @@ def test_graph_replay_with_new_data(torch_mod):
                 p.run(x, o, out, total=total, stream=stream)
             stream.synchronize()
             _check(out.cpu().numpy(), a, off, False, "graph replay %d" % rep)
+
+
+def _c4_batch(seed=2024):
+    """The batched configuration at its full size: 8192 segments, lengths
+    floor(exp(U(ln 64, ln 2^20))) (numpy seed), class per segment uniform
+    over random / strictly descending / strictly ascending, int64 CSR
+    offsets; about 8.7e8 keys."""
+    rng = np.random.default_rng(seed)
+    nseg = 8192
+    lengths = np.floor(np.exp(rng.uniform(np.log(64), np.log(1 << 20), size=nseg))).astype(np.int64)
+    cls = rng.integers(0, 3, size=nseg)
+    off = _offsets(lengths)
+    a = np.empty(int(off[-1]), dtype=np.int32)
+    kinds = ("random", "descending", "ascending")
+    for k in range(nseg):
+        oracle.fill(kinds[cls[k]], int(lengths[k]), seed=seed + k, out=a[off[k]:off[k + 1]])
+    return a, off
+
+
+def test_c4_full_batch_eager_and_graph(torch_mod):
+    """The full batched configuration in one plan, checked bit-exact over
+    the whole batch against the oracle's per-segment stack: direct launches
+    in both tile shapes, then the autotuner's choice replayed as a hipGraph
+    (twice, with the answers poisoned between replays)."""
+    import os
+    import apsep_amd as A
+    torch = torch_mod
+    a, off = _c4_batch()
+    total, nseg = a.shape[0], off.shape[0] - 1
+    _log("C4 batch", nseg, "segments,", total, "keys")
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    exp = oracle.apsep_segmented(a, off, False, threads=threads)
+    x = torch.from_numpy(a).cuda()
+    o = torch.from_numpy(off).cuda()
+    out = torch.empty(total, dtype=torch.int32, device="cuda")
+
+    def check(what):
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        if not np.array_equal(got, exp):
+            bad = np.nonzero(got != exp)[0]
+            i = int(bad[0])
+            k = int(np.searchsorted(off, i, side="right") - 1)
+            raise AssertionError("C4 %s: %d mismatches, first i=%d (segment %d) got=%d exp=%d"
+                                 % (what, bad.size, i, k, got[i], exp[i]))
+
+    stream = torch.cuda.Stream()
+    with A.BatchPlan(total, nseg) as p, torch.cuda.stream(stream):
+        for lanes in (256, 512):
+            p.set_tile_lanes(lanes)
+            out.fill_(int(POISON))
+            _log("C4 eager", lanes)
+            p.run(x, o, out, stream=stream)
+            check("eager lanes=%d" % lanes)
+        p.tune(x, o, out, stream=stream)
+        check("autotuned")
+        p.use_graph(True)
+        for rep in range(2):
+            out.fill_(int(POISON))
+            _log("C4 graph replay", rep, "lanes", p.tile_lanes)
+            p.run(x, o, out, stream=stream)
+            check("graph replay %d lanes=%d" % (rep, p.tile_lanes))
+    del x, out
+    torch.cuda.empty_cache()

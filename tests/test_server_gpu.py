@@ -189,3 +189,60 @@ def test_bulk_submit_and_release(torch_mod):
         big = torch.zeros(1 << 22, dtype=torch.int32, device="cuda")
         assert s.submit(big.data_ptr(), 1 << 22) is not None
         s.drain()
+
+
+def test_c5_served_stream(torch_mod):
+    """The served configuration: Poisson arrivals, lengths log-uniform in
+    [2^10, 2^24] (numpy seed), classes 60/20/20 random / descending /
+    ascending.  The pool is sized so that admission refuses jobs while the
+    early ones run (a refused job waits in the client's backlog and is
+    resubmitted after a poll); every job's answers are checked against the
+    oracle."""
+    import time
+    import apsep_amd as A
+    torch = torch_mod
+    rng = np.random.default_rng(5)
+    J = 256
+    lengths = np.floor(np.exp(rng.uniform(np.log(1 << 10), np.log(1 << 24), size=J))).astype(np.int64)
+    cls = rng.choice(3, size=J, p=[0.6, 0.2, 0.2])
+    kinds = ("random", "descending", "ascending")
+    host = [oracle.fill(kinds[cls[j]], int(lengths[j]), seed=500 + j) for j in range(J)]
+    dev = [torch.from_numpy(h).cuda() for h in host]
+    torch.cuda.synchronize()
+    # offered jobs/s: the whole stream (about 4e8 keys) arrives within a few
+    # milliseconds, faster than it is served, so the pool fills up
+    rate = 100000.0
+    arrive = np.cumsum(rng.exponential(1.0 / rate, size=J))
+    pool = 256 << 20  # 128 MB of keys: room for a few of the largest jobs at once
+    assert pool // 2 >= 4 * int(lengths.max())
+    _log("C5 stream", J, "jobs,", int(lengths.sum()), "keys, pool", pool)
+    ids = [None] * J
+    got = [None] * J
+    refused = 0
+    with A.Server(pool, 1 << 26, 1024) as s:
+        t0 = time.monotonic()
+        nxt = done = 0
+        while done < J:
+            now = time.monotonic() - t0
+            assert now < 300, "served stream stalled (%d of %d jobs done)" % (done, J)
+            while nxt < J and arrive[nxt] <= now:
+                jid = s.submit(dev[nxt].data_ptr(), int(lengths[nxt]))
+                if jid is None:
+                    refused += 1
+                    break  # backlog: retry after the next poll
+                ids[nxt] = jid
+                nxt += 1
+            s.poll()
+            # finished jobs hand their answers back and free their slots
+            while done < nxt and s.info(ids[done]).state == A.JOB_DONE:
+                info = s.info(ids[done])
+                assert info.n == lengths[done]
+                got[done] = _answers(s, ids[done])
+                s.release(ids[done])
+                done += 1
+    assert refused >= 1, "the pool budget never refused a job"
+    _log("C5 refused", refused, "submissions")
+    for j in range(J):
+        if not np.array_equal(got[j], oracle.apsep(host[j])):
+            raise AssertionError("C5 job %d (n=%d, %s) differs from the oracle"
+                                 % (j, lengths[j], kinds[cls[j]]))
