@@ -34,7 +34,7 @@ ALGOS = {"wstl": ALGO_WSTL, "spt": ALGO_SPT, "lookback": ALGO_LOOKBACK,
 EXPORTS = [
     "apsep_status_string", "apsep_version", "apsep_scratch_create",
     "apsep_scratch_destroy", "apsep_scratch_bytes", "apsep_run",
-    "apsep_launch", "apsep_run_mode", "apsep_scratch_query",
+    "apsep_launch", "apsep_run_mode", "apsep_run_noalloc", "apsep_scratch_query",
     "apsep_scratch_check", "apsep_time_kernels",
     "apsep_batch_plan_create", "apsep_batch_plan_destroy", "apsep_run_segmented",
     "apsep_batch_plan_use_graph", "apsep_batch_plan_tune",
@@ -98,6 +98,7 @@ def lib() -> ctypes.CDLL:
         L.apsep_run.argtypes = [vp, vp, vp, i64, vp]
         L.apsep_launch.argtypes = [ci, vp, vp, i64, ci]
         L.apsep_run_mode.argtypes = [vp, vp, vp, i64, ci, vp]
+        L.apsep_run_noalloc.argtypes = [vp, vp, i64, ci, vp]
         L.apsep_scratch_query.argtypes = [vp, ctypes.POINTER(ScratchInfo)]
         L.apsep_scratch_check.argtypes = [vp, ctypes.POINTER(i32)]
         L.apsep_time_kernels.argtypes = [vp, vp, vp, i64, ci, ci, vp,
@@ -493,6 +494,24 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+NOALLOC_MAX_N = 1 << 26  # APSEP_NOALLOC_MAX_N (include/apsep/apsep.h)
+
+
+def run_noalloc(x, out, n: int | None = None, incl: bool = False, stream=None):
+    """Solve ``x`` into ``out`` without any device allocation
+    (apsep_run_noalloc: the scratch is the library's fixed per-GPU array, n
+    <= NOALLOC_MAX_N; calls that overlap on one GPU race)."""
+    import torch
+    if n is None:
+        n = x.numel()
+    check_single_args(x, out, n, NOALLOC_MAX_N, _device_index(x))
+    if stream is None:
+        stream = torch.cuda.current_stream(x.device)
+    _check(lib().apsep_run_noalloc(x.data_ptr(), out.data_ptr(), n, int(incl),
+                                   stream.cuda_stream), "apsep_run_noalloc")
+    return out
 
 
 def launch(algo, d_in: int, d_out: int, n: int, incl: bool = False):

@@ -161,6 +161,16 @@ apsep_status apsep_run_mode(apsep_scratch* s, const int32_t* d_in, int32_t* d_ou
     return single_run(s, d_in, d_out, (int)n, incl, (hipStream_t)stream);
 }
 
+apsep_status apsep_run_noalloc(const int32_t* d_in, int32_t* d_out, int64_t n, int incl,
+                              void* stream) {
+    if (incl != 0 && incl != 1) return APSEP_ERR_INVALID_ARG;
+    if (n <= 0) return APSEP_OK;
+    if (n > APSEP_NOALLOC_MAX_N) return APSEP_ERR_CAPACITY;
+    const apsep_status st = check_io(d_in, d_out, n);
+    if (st != APSEP_OK) return st;
+    return noalloc_run(d_in, d_out, (int)n, incl, (hipStream_t)stream);
+}
+
 apsep_status apsep_time_kernels(apsep_scratch* s, const int32_t* d_in, int32_t* d_out,
                                 int64_t n, int incl, int reps, void* stream,
                                 apsep_kernel_times* out) {

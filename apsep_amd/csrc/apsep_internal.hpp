@@ -67,6 +67,8 @@ apsep_status single_scratch_init(apsep_scratch* s);
 apsep_status single_run(apsep_scratch* s, const int32_t* in, int32_t* out, int n,
                         int incl, hipStream_t st);
 apsep_status single_error(apsep_scratch* s, int* code);
+// apsep_run_noalloc's solve: scratch in a per-device file-scope array
+apsep_status noalloc_run(const int32_t* in, int32_t* out, int n, int incl, hipStream_t st);
 // The segmented solver, built for tiles of 256 and of 512 lanes
 // (apsep_segmented_256.hip, apsep_segmented_512.hip).  seg_tile_elems is
 // that build's tile in elements.

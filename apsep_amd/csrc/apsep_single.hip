@@ -11,6 +11,7 @@
 #include <stdint.h>
 #include <string.h>
 #include <stdlib.h>
+#include <mutex>
 #include <new>
 
 #include "apsep_device.hpp"
@@ -303,11 +304,13 @@ static apsep_status fill_levels(apsep_scratch* s, int nt, Levels& L, hipStream_t
     return APSEP_OK;
 }
 
+// fixed_epoch: every run takes the path of a captured run (level-1 reset,
+// CAPTURE_EPOCH) and the scratch keeps no state between runs.
 template <bool INCL>
 static apsep_status run_wstl(apsep_scratch* s, const int* in, int* out, int n,
-                             hipStream_t st) {
+                             hipStream_t st, bool fixed_epoch = false) {
     const int nt = (int)(((long long)n + TILE - 1) / TILE);  // (n + TILE - 1 overflows near 2^31)
-    const int cap = capture_state(st);
+    const int cap = fixed_epoch ? 1 : capture_state(st);
     if (cap < 0) return APSEP_ERR_UNSUPPORTED;
     Levels L;
     if (fill_levels(s, nt, L, st, cap) != APSEP_OK) return APSEP_ERR_LAUNCH;
@@ -425,6 +428,66 @@ apsep_status single_run(apsep_scratch* s, const int32_t* in, int32_t* out, int n
     default:
         return APSEP_ERR_INVALID_ARG;
     }
+}
+
+// ---------------------------------------------------------------------------
+// The no-allocation entry (apsep_run_noalloc): the multi-launch solve with
+// its scratch in `scratch`, a file-scope device array sized for
+// NOALLOC_MAX_N keys, which every GPU the library runs on holds until the
+// process ends.  In host code `scratch` names a host-side stand-in: its
+// device address comes from hipGetSymbolAddress, once per device with that
+// device current, and only that address is used (kernel arguments).  Every
+// run empties level 1 and uses the fixed epoch, so no state carries from one
+// call to the next and nothing needs clearing at set-up; calls that overlap
+// on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+constexpr int64_t NOALLOC_MAX_N = APSEP_NOALLOC_MAX_N;
+constexpr int64_t NOALLOC_BYTES = common_scratch_bytes(NOALLOC_MAX_N, TILE) + ALIGN_SLACK;
+static_assert(NOALLOC_MAX_N <= INT32_MAX, "int32 indices");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static apsep_scratch g_noalloc[NOALLOC_MAX_DEVICES];  // views of each device's array
+static bool g_noalloc_ready[NOALLOC_MAX_DEVICES];
+
+// Per-device set-up on the first call: the array's device address, carved
+// into the solver's arrays.  Host-side bookkeeping only; no allocation.
+static apsep_status noalloc_setup(int dev) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ready[dev]) return APSEP_OK;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return APSEP_ERR_DEVICE;
+    }
+    apsep_scratch& s = g_noalloc[dev];
+    memset(&s, 0, sizeof s);
+    s.algo = APSEP_ALGO_WSTL;
+    s.n_cap = NOALLOC_MAX_N;
+    s.device = dev;
+    s.tile = TILE;
+    s.nt_cap = (int)((NOALLOC_MAX_N + TILE - 1) / TILE);
+    s.base = ptr;
+    s.bytes = NOALLOC_BYTES;
+    if (!carve_arrays(&s)) return APSEP_ERR_ALLOC;  // sizing bug guard
+    s.epoch = CAPTURE_EPOCH;
+    g_noalloc_ready[dev] = true;
+    return APSEP_OK;
+}
+
+apsep_status noalloc_run(const int32_t* in, int32_t* out, int n, int incl, hipStream_t st) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) {
+        (void)hipGetLastError();
+        return APSEP_ERR_DEVICE;
+    }
+    if (dev >= NOALLOC_MAX_DEVICES) return APSEP_ERR_DEVICE;
+    const apsep_status r = noalloc_setup(dev);
+    if (r != APSEP_OK) return r;
+    apsep_scratch* s = &g_noalloc[dev];
+    return incl ? run_wstl<true>(s, in, out, n, st, true) : run_wstl<false>(s, in, out, n, st, true);
 }
 
 apsep_status single_error(apsep_scratch* s, int* code) {

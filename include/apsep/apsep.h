@@ -93,6 +93,21 @@ apsep_status apsep_launch(apsep_algo algo, const int32_t* d_in, int32_t* d_out,
 apsep_status apsep_run_mode(apsep_scratch* s, const int32_t* d_in,
                             int32_t* d_out, int64_t n, int incl, void* stream);
 
+/* Solve without allocating: no counterpart in the reference.  For callers
+ * that cannot call hipMalloc (code under graph capture, tight serving loops,
+ * harnesses that forbid allocation).  The same answers as apsep_run_mode
+ * (multi-launch solver, predicate per call), with the scratch in a fixed
+ * per-GPU array of the library (about 8.2 MB, held by every GPU the library
+ * runs on until the process ends) instead of an apsep_scratch.  Asynchronous
+ * on `stream`; n <= 0 is a no-op; n > APSEP_NOALLOC_MAX_N ->
+ * APSEP_ERR_CAPACITY.  Calls on one GPU share the array: calls that overlap
+ * on the device (on different streams, or from different threads) race and
+ * give unspecified answers.  It keeps no state between calls, so it may be
+ * recorded into a graph and replayed. */
+#define APSEP_NOALLOC_MAX_N 67108864
+apsep_status apsep_run_noalloc(const int32_t* d_in, int32_t* d_out, int64_t n, int incl,
+                               void* stream);
+
 /* Shape of a scratch: the counterpart of the fields of WSTLScratch /
  * SPTScratch (src/apsep.cuh:2509-2518, 3001-3011). */
 typedef struct apsep_scratch_info {
