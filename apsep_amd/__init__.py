@@ -32,10 +32,10 @@ ALGOS = {"wstl": ALGO_WSTL, "spt": ALGO_SPT, "lookback": ALGO_LOOKBACK,
 
 # every symbol include/apsep/apsep.h declares
 EXPORTS = [
-    "apsep_status_string", "apsep_version", "apsep_scratch_create",
+    "apsep_status_string", "apsep_version", "apsep_last_error", "apsep_scratch_create",
     "apsep_scratch_destroy", "apsep_scratch_bytes", "apsep_run",
     "apsep_launch", "apsep_run_mode", "apsep_run_noalloc", "apsep_scratch_query",
-    "apsep_scratch_check", "apsep_time_kernels",
+    "apsep_scratch_check", "apsep_time_kernels", "apsep_time_kernels_rotating",
     "apsep_batch_plan_create", "apsep_batch_plan_destroy", "apsep_run_segmented",
     "apsep_batch_plan_use_graph", "apsep_batch_plan_tune",
     "apsep_batch_plan_set_tile_lanes", "apsep_batch_plan_tile_lanes",
@@ -77,7 +77,9 @@ class ApsepError(RuntimeError):
     def __init__(self, status: int, what: str = ""):
         self.status = status
         msg = lib().apsep_status_string(status).decode()
-        super().__init__("%s%s (status %d)" % (what + ": " if what else "", msg, status))
+        detail = lib().apsep_last_error().decode()
+        super().__init__("%s%s (status %d; %s)" % (what + ": " if what else "", msg, status,
+                                                   detail))
 
 
 def lib() -> ctypes.CDLL:
@@ -92,6 +94,7 @@ def lib() -> ctypes.CDLL:
         L.apsep_status_string.argtypes = [ci]
         L.apsep_status_string.restype = ctypes.c_char_p
         L.apsep_version.restype = ctypes.c_char_p
+        L.apsep_last_error.restype = ctypes.c_char_p
         L.apsep_scratch_create.argtypes = [ci, i64, ci, ctypes.POINTER(vp)]
         L.apsep_scratch_destroy.argtypes = [vp]
         L.apsep_scratch_bytes.argtypes = [ci, i64, ci, ctypes.POINTER(i64)]
@@ -103,6 +106,8 @@ def lib() -> ctypes.CDLL:
         L.apsep_scratch_check.argtypes = [vp, ctypes.POINTER(i32)]
         L.apsep_time_kernels.argtypes = [vp, vp, vp, i64, ci, ci, vp,
                                          ctypes.POINTER(KernelTimes)]
+        L.apsep_time_kernels_rotating.argtypes = [vp, vp, vp, i32, i64, ci, ci, vp,
+                                                  ctypes.POINTER(KernelTimes)]
         L.apsep_batch_plan_create.argtypes = [i64, i32, ci, ctypes.POINTER(vp)]
         L.apsep_batch_plan_destroy.argtypes = [vp]
         L.apsep_run_segmented.argtypes = [vp, vp, vp, vp, i32, i64, vp]
@@ -125,7 +130,7 @@ def lib() -> ctypes.CDLL:
         L.apsep_server_job_times.argtypes = [vp, i64, i32, vp, vp]
         L.apsep_server_release_upto.argtypes = [vp, i64]
         for f in EXPORTS:
-            if f != "apsep_status_string" and f != "apsep_version":
+            if f not in ("apsep_status_string", "apsep_version", "apsep_last_error"):
                 getattr(L, f).restype = ci
         _lib = L
     return _lib
@@ -512,6 +517,33 @@ def run_noalloc(x, out, n: int | None = None, incl: bool = False, stream=None):
     _check(lib().apsep_run_noalloc(x.data_ptr(), out.data_ptr(), n, int(incl),
                                    stream.cuda_stream), "apsep_run_noalloc")
     return out
+
+
+def time_kernels_rotating(scratches, xs, outs, reps: int, n: int | None = None,
+                          stream=None, incl: bool = False) -> KernelTimes:
+    """Each kernel of the solve launched ``reps`` times back to back, launch
+    r solving array r % len(xs) with its own scratch, input and output
+    (apsep_time_kernels_rotating); returns the summed device time per
+    kernel (blocks)."""
+    import torch
+    m = len(xs)
+    if not (len(scratches) == len(outs) == m) or not 1 <= m <= 8:
+        raise ValueError("apsep: 1..8 (scratch, input, output) triples required")
+    if n is None:
+        n = xs[0].numel()
+    for sc, x, o in zip(scratches, xs, outs):
+        check_single_args(x, o, n, sc.n_cap, sc.device)
+    if stream is None:
+        stream = torch.cuda.current_stream(xs[0].device)
+    P = ctypes.c_void_p * m
+    hs = P(*[sc._h.value for sc in scratches])
+    ins = P(*[x.data_ptr() for x in xs])
+    ous = P(*[o.data_ptr() for o in outs])
+    t = KernelTimes()
+    _check(lib().apsep_time_kernels_rotating(hs, ins, ous, m, n, int(incl), reps,
+                                             stream.cuda_stream, ctypes.byref(t)),
+           "apsep_time_kernels_rotating")
+    return t
 
 
 def launch(algo, d_in: int, d_out: int, n: int, incl: bool = False):

@@ -4,6 +4,7 @@
 // on a machine without a GPU.  Nothing here prints or exits.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <string.h>
 #include <new>
 
@@ -13,7 +14,26 @@
 
 using namespace apsep;
 
+namespace apsep {
+
+// The calling thread's last recorded failure (apsep_last_error).
+static thread_local char g_last_error[192];
+
+void record_hip_error(hipError_t e, const char* func, int line) {
+    snprintf(g_last_error, sizeof g_last_error, "%s (%s) in %s, line %d", hipGetErrorName(e),
+             hipGetErrorString(e), func, line);
+}
+
+static void record_device_error(int code) {
+    snprintf(g_last_error, sizeof g_last_error,
+             "device-side error word %d: a grid barrier or look-back wait gave up", code);
+}
+
+}  // namespace apsep
+
 extern "C" {
+
+const char* apsep_last_error(void) { return g_last_error[0] ? g_last_error : "none"; }
 
 const char* apsep_status_string(apsep_status s) {
     switch (s) {
@@ -67,7 +87,7 @@ static apsep_status scratch_create(apsep_algo algo, int64_t n_cap, int incl, int
     int dev = -1, ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 ||
         hipGetDevice(&dev) != hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_DEVICE;
     }
     apsep_scratch* s = new (std::nothrow) apsep_scratch();  // zeroed
@@ -80,7 +100,7 @@ static apsep_status scratch_create(apsep_algo algo, int64_t n_cap, int incl, int
     s->device = dev;
     if (hipDeviceGetAttribute(&s->num_cus, hipDeviceAttributeMultiprocessorCount, dev) !=
         hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         delete s;
         return APSEP_ERR_DEVICE;
     }
@@ -155,7 +175,7 @@ apsep_status apsep_run_mode(apsep_scratch* s, const int32_t* d_in, int32_t* d_ou
     if (st != APSEP_OK) return st;
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess || dev != s->device) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_DEVICE;
     }
     return single_run(s, d_in, d_out, (int)n, incl, (hipStream_t)stream);
@@ -182,11 +202,44 @@ apsep_status apsep_time_kernels(apsep_scratch* s, const int32_t* d_in, int32_t* 
     if (n > s->n_cap) return APSEP_ERR_CAPACITY;
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess || dev != s->device) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_DEVICE;
     }
     double ms[3] = {0, 0, 0};
-    st = single_time_kernels(s, d_in, d_out, (int)n, incl, reps, (hipStream_t)stream, ms);
+    st = single_time_kernels(&s, &d_in, &d_out, 1, (int)n, incl, reps, (hipStream_t)stream, ms);
+    if (st != APSEP_OK) return st;
+    out->runs = reps;
+    out->tile_ms = ms[0];
+    out->levels_ms = ms[1];
+    out->fix_ms = ms[2];
+    return APSEP_OK;
+}
+
+apsep_status apsep_time_kernels_rotating(apsep_scratch* const* s, const int32_t* const* d_in,
+                                         int32_t* const* d_out, int32_t count, int64_t n,
+                                         int incl, int reps, void* stream,
+                                         apsep_kernel_times* out) {
+    if (!s || !d_in || !d_out || !out || count < 1 || count > 8 || reps < 1 ||
+        (incl != 0 && incl != 1))
+        return APSEP_ERR_INVALID_ARG;
+    memset(out, 0, sizeof *out);
+    if (n <= 0) return APSEP_OK;
+    int dev = -1;
+    for (int k = 0; k < count; k++) {
+        if (!s[k] || s[k]->algo != s[0]->algo) return APSEP_ERR_INVALID_ARG;
+        const apsep_status st = check_io(d_in[k], d_out[k], n);
+        if (st != APSEP_OK) return st;
+        if (n > s[k]->n_cap) return APSEP_ERR_CAPACITY;
+    }
+    if (hipGetDevice(&dev) != hipSuccess) {
+        APSEP_CLEAR_HIP_ERROR();
+        return APSEP_ERR_DEVICE;
+    }
+    for (int k = 0; k < count; k++)
+        if (s[k]->device != dev) return APSEP_ERR_DEVICE;
+    double ms[3] = {0, 0, 0};
+    const apsep_status st = single_time_kernels(s, d_in, d_out, count, (int)n, incl, reps,
+                                                (hipStream_t)stream, ms);
     if (st != APSEP_OK) return st;
     out->runs = reps;
     out->tile_ms = ms[0];
@@ -219,7 +272,7 @@ apsep_status apsep_batch_plan_create(int64_t total_cap, int32_t nseg_cap, int in
     const int64_t tiles = (total_cap + tile - 1) / tile;
     if (tiles > 0 && (hipMalloc((void**)&p->tmap, (size_t)tiles * sizeof(int)) != hipSuccess ||
                       hipMalloc((void**)&p->tbound, (size_t)tiles * sizeof(int2)) != hipSuccess)) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         if (p->tmap) (void)hipFree(p->tmap);
         apsep_scratch_destroy(p->s);
         delete p;
@@ -285,7 +338,7 @@ apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
     if ((uintptr_t)d_offsets & 7) return APSEP_ERR_ALIGN;
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess || dev != p->s->device) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_DEVICE;
     }
     hipStream_t hs = (hipStream_t)stream;
@@ -297,20 +350,20 @@ apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
     if (!same) {
         drop_graph(p);
         if (hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-            (void)hipGetLastError();
+            APSEP_CLEAR_HIP_ERROR();
             return APSEP_ERR_LAUNCH;
         }
         st = plan_run(p, d_in, d_offsets, d_out, nseg, (int)total, hs);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(hs, &g);
         if (st != APSEP_OK || ec != hipSuccess || !g) {
-            (void)hipGetLastError();
+            APSEP_CLEAR_HIP_ERROR();
             if (g) (void)hipGraphDestroy(g);
             return APSEP_ERR_LAUNCH;
         }
         p->graph = g;
         if (hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) != hipSuccess) {
-            (void)hipGetLastError();
+            APSEP_CLEAR_HIP_ERROR();
             drop_graph(p);
             return APSEP_ERR_LAUNCH;
         }
@@ -322,7 +375,7 @@ apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
         p->g_stream = stream;
     }
     if (hipGraphLaunch(p->exec, hs) != hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_LAUNCH;
     }
     return APSEP_OK;
@@ -337,11 +390,11 @@ apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
     hipStream_t hs = (hipStream_t)stream;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_DEVICE;
     }
     if (hipEventCreate(&e1) != hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         (void)hipEventDestroy(e0);
         return APSEP_ERR_DEVICE;
     }
@@ -368,7 +421,7 @@ apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st != APSEP_OK) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return st;
     }
     int bk = 0, bg = 0;
@@ -394,13 +447,16 @@ apsep_status apsep_launch(apsep_algo algo, const int32_t* d_in, int32_t* d_out,
     if (st != APSEP_OK) return st;
     st = apsep_run(s, d_in, d_out, n, nullptr);
     if (st == APSEP_OK && hipDeviceSynchronize() != hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         st = APSEP_ERR_LAUNCH;
     }
     if (st == APSEP_OK) {
         int32_t derr = 0;
         st = apsep_scratch_check(s, &derr);
-        if (st == APSEP_OK && derr) st = APSEP_ERR_LAUNCH;
+        if (st == APSEP_OK && derr) {
+            record_device_error(derr);
+            st = APSEP_ERR_LAUNCH;
+        }
     }
     apsep_scratch_destroy(s);
     return st;

@@ -76,13 +76,19 @@ constexpr int XQ = 3;              // further queries per lane tried on the prev
 // segmented solves try them only with 256-lane tiles (run at 5 waves per
 // SIMD, room to spare); 512-lane segmented tiles run at 6 and would spill
 constexpr int XQ_SEG = NT == 256 ? XQ : 0;
+#ifndef APSEP_XQ_SPT
+#define APSEP_XQ_SPT 2
+#endif
+// the persistent launch's tile loop holds a few more registers live: with
+// XQ_SPT checks its tile phase fits 80 VGPRs without spilling to memory
+constexpr int XQ_SPT = APSEP_XQ_SPT;
 constexpr int FAN = 64;            // hierarchy fan-out
 constexpr int LOG_FAN = 6;
 constexpr int MAX_LEVELS = 5;      // 64^5 tiles >> 2^31 / TILE
 constexpr int UPPER_MIN = 512;     // level-1 size above which levels >= 2 are stored
 constexpr int K1_LOG = 5;          // level-1 keys 2^5 u64 (256 B) apart (see key_at)
 constexpr int INF = INT_MAX;
-#define SPT_WAVES_PER_EU 4         // occupancy the persistent kernel is pinned at
+#define SPT_WAVES_PER_EU 6         // occupancy the persistent kernel is pinned at
 
 static_assert(K == 16 && LPC == 4 && (1 << LOG_LPC) == LPC, "tile shape");
 static_assert(NWAVE == 4 || NWAVE == 8, "the wave searches handle 4 or 8 waves per tile");
@@ -123,6 +129,67 @@ __device__ __forceinline__ u64 make_key(unsigned epoch, int v) {
 }
 __device__ __forceinline__ int key_value(u64 k, unsigned epoch) {
     return ((unsigned)(k >> 32) == ~epoch) ? (int)((unsigned)k ^ 0x80000000u) : INF;
+}
+
+// Accesses to data that crosses workgroups inside one persistent launch
+// (CO = true).  Each XCD of an MI355X has its own L2, and a plain store
+// stays in the writer's L2 until it is written back, so another XCD may
+// read the old value; a fence that writes the L2 back before a grid barrier
+// costs ~30 us at N = 2^25 (it writes back every dirty output line too).
+// Instead, the few summary words that other workgroups read are stored and
+// loaded as agent-scope relaxed atomics, which the compiler emits as sc1
+// (device-coherent) accesses, and the grid barrier only waits for every
+// lane's memory operations to complete (grid_barrier_co).  CO = false: plain
+// accesses, for data read only after a kernel boundary or by the writer's
+// own workgroup.
+// Waits for the calling lane's memory operations to complete; the "memory"
+// clobber also keeps the compiler from moving any of them past the wait.
+__device__ __forceinline__ void wait_own_memory_ops() {
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+}
+
+template <bool CO>
+__device__ __forceinline__ int ld_i(const int* p) {
+    if (CO) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return *p;
+}
+template <bool CO>
+__device__ __forceinline__ u64 ld_u64(const u64* p) {
+    if (CO) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return *p;
+}
+template <bool CO>
+__device__ __forceinline__ int2 ld_i2(const int2* p) {
+    if (CO) {
+        const u64 w = ld_u64<true>(reinterpret_cast<const u64*>(p));
+        return make_int2((int)(unsigned)w, (int)(unsigned)(w >> 32));
+    }
+    return *p;
+}
+template <bool CO>
+__device__ __forceinline__ int4 ld_i4(const int4* p) {
+    if (CO) {
+        const u64* q = reinterpret_cast<const u64*>(p);
+        const u64 a = ld_u64<true>(q), b = ld_u64<true>(q + 1);
+        return make_int4((int)(unsigned)a, (int)(unsigned)(a >> 32), (int)(unsigned)b,
+                         (int)(unsigned)(b >> 32));
+    }
+    return *p;
+}
+template <bool CO>
+__device__ __forceinline__ void st_i(int* p, int v) {
+    if (CO) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+template <bool CO>
+__device__ __forceinline__ void st_u64(u64* p, u64 v) {
+    if (CO) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+template <bool CO>
+__device__ __forceinline__ void st_i2(int2* p, int2 v) {
+    if (CO) st_u64<true>(reinterpret_cast<u64*>(p), (u64)(unsigned)v.x | ((u64)(unsigned)v.y << 32));
+    else *p = v;
 }
 
 // Per-tile summaries written by the tile pass.
@@ -226,6 +293,12 @@ __device__ __forceinline__ int dpp0(int x) {
     return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, true);
 }
 __device__ __forceinline__ int readlane(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+// The lane's index in its wave, computed (bits of an all-ones mask below the
+// lane), not read from the work-item id: a persistent kernel that needs the
+// id across a long loop would otherwise hold it in a register throughout.
+__device__ __forceinline__ int lane_id() {
+    return (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
 __device__ __forceinline__ int lane63(int x) { return __builtin_amdgcn_readlane(x, WAVE - 1); }
 
 // Inclusive min scan over the wave (Hillis-Steele), lanes below the shift
@@ -257,13 +330,13 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Rightmost index in [lo, hi] with qual(arr[idx], v), or -1.  Reads aligned
 // dwordx4 groups; arr must be readable (padded) up to (hi | 3).
-template <bool INCL>
+template <bool INCL, bool CO = false>
 __device__ __forceinline__ int scan_back(const int* __restrict__ arr, int lo,
                                          int hi, int v) {
     int i = hi;
     while (i >= lo) {
         const int a4 = i & ~3;
-        const int4 w = *reinterpret_cast<const int4*>(arr + a4);
+        const int4 w = ld_i4<CO>(reinterpret_cast<const int4*>(arr + a4));
         if (a4 + 3 <= i && a4 + 3 >= lo && qual<INCL>(w.w, v)) return a4 + 3;
         if (a4 + 2 <= i && a4 + 2 >= lo && qual<INCL>(w.z, v)) return a4 + 2;
         if (a4 + 1 <= i && a4 + 1 >= lo && qual<INCL>(w.y, v)) return a4 + 1;
@@ -274,20 +347,20 @@ __device__ __forceinline__ int scan_back(const int* __restrict__ arr, int lo,
 }
 
 // Same over an epoch-keyed level.
-template <bool INCL>
+template <bool INCL, bool CO = false>
 __device__ __forceinline__ int scan_back_key(const Levels& L, int lvl, int lo, int hi, int v) {
     for (int i = hi; i >= lo; i--)
-        if (qual<INCL>(key_value(key_at(L, lvl, i), L.epoch), v)) return i;
+        if (qual<INCL>(key_value(ld_u64<CO>(&key_at(L, lvl, i)), L.epoch), v)) return i;
     return -1;
 }
 
 // Rightmost qualifying index in the aligned 32-entry block [b, b+32) (b a
 // multiple of 4), all loads issued together; -1 if none.
-template <bool INCL>
+template <bool INCL, bool CO = false>
 __device__ __forceinline__ int last_in_32(const int* __restrict__ arr, int b, int v) {
     int4 w[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) w[k] = *reinterpret_cast<const int4*>(arr + b + 4 * k);
+    for (int k = 0; k < 8; k++) w[k] = ld_i4<CO>(reinterpret_cast<const int4*>(arr + b + 4 * k));
     int r = -1;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
@@ -302,16 +375,16 @@ __device__ __forceinline__ int last_in_32(const int* __restrict__ arr, int b, in
 // Nearest tile s < t with qual(tile min, v); the caller guarantees one
 // exists (some tile before t qualifies).  Ascends the 64-ary hierarchy until an
 // earlier entry qualifies, then descends to the tile.
-template <bool INCL>
+template <bool INCL, bool CO = false>
 __device__ int find_tile(const Levels& L, int t, int v) {
-    int found = scan_back<INCL>(L.mn0, (t - 1) & ~(FAN - 1), t - 1, v);
+    int found = scan_back<INCL, CO>(L.mn0, (t - 1) & ~(FAN - 1), t - 1, v);
     if (found >= 0) return found;
     int x = (t - 1) >> LOG_FAN;  // exclusive bound at level 1
     if (L.nlev < 2 || x <= 0) return -1;  // unreachable when guaranteed
     int lvl = 1;
     if (L.upper) {
         while (true) {
-            found = scan_back_key<INCL>(L, lvl, (x - 1) & ~(FAN - 1), x - 1, v);
+            found = scan_back_key<INCL, CO>(L, lvl, (x - 1) & ~(FAN - 1), x - 1, v);
             if (found >= 0) break;
             x = (x - 1) >> LOG_FAN;
             lvl++;
@@ -320,14 +393,14 @@ __device__ int find_tile(const Levels& L, int t, int v) {
         while (lvl > 1) {
             lvl--;
             const int lo = found << LOG_FAN;
-            found = scan_back_key<INCL>(L, lvl, lo, min(lo + FAN - 1, L.n[lvl] - 1), v);
+            found = scan_back_key<INCL, CO>(L, lvl, lo, min(lo + FAN - 1, L.n[lvl] - 1), v);
         }
     } else {
-        found = scan_back_key<INCL>(L, 1, 0, x - 1, v);
+        found = scan_back_key<INCL, CO>(L, 1, 0, x - 1, v);
         if (found < 0) return -1;
     }
     const int lo = found << LOG_FAN;
-    return scan_back<INCL>(L.mn0, lo, min(lo + FAN - 1, L.n[0] - 1), v);
+    return scan_back<INCL, CO>(L.mn0, lo, min(lo + FAN - 1, L.n[0] - 1), v);
 }
 
 // Rightmost j in [0, K) with qual(lane lt's value j, v), or -1.
@@ -404,8 +477,9 @@ __device__ __forceinline__ int intile_answer(const TileSmem& sm, int qt, int val
 // `sm` may be reused for the next tile right away.
 // ---------------------------------------------------------------------------
 // This lane's K values of tile t (INF past the end of the array).
-__device__ __forceinline__ void load_lane(const int* __restrict__ in, int n, int t, int* v) {
-    const int base = t * TILE + (int)threadIdx.x * K;
+__device__ __forceinline__ void load_lane(const int* __restrict__ in, int n, int t, int* v,
+                                          int tid = (int)threadIdx.x) {
+    const int base = t * TILE + tid * K;
     if (t * TILE <= n - TILE) {  // (not t * TILE + TILE: it overflows near 2^31)
 #pragma unroll
         for (int q = 0; q < K4; q++) {
@@ -518,12 +592,12 @@ __device__ __forceinline__ unsigned qual_bits(int p, const int* v) {
     return INCL ? (~m & 0xFFFFu) : m;
 }
 
-template <bool INCL, bool SEG = false, bool L1 = true>
+template <bool INCL, bool SEG = false, bool L1 = true, bool CO = false>
 __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, const int* v,
                                            int prev_last, TileSmem& sm, const TileOut& to,
                                            const Levels& L,
-                                           const SegLane sl = SegLane{0, 0u}) {
-    const int tid = threadIdx.x;
+                                           const SegLane sl = SegLane{0, 0u},
+                                           const int tid = (int)threadIdx.x) {
     const int lane = tid & (WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar tests
     const int row = lane >> 4;
@@ -569,7 +643,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     sm.win[2][tid] = w4;
     sm.win[3][tid] = w8;
     // window of LPC lanes = one 64-element chunk
-    if ((lane & (LPC - 1)) == LPC - 1) to.cmin[(size_t)t * CPT + (tid >> LOG_LPC)] = w4;
+    if ((lane & (LPC - 1)) == LPC - 1) st_i<CO>(&to.cmin[(size_t)t * CPT + (tid >> LOG_LPC)], w4);
     int winc = min(w16, dpp<DPP_ROW_BCAST15, 0xa>(INF, w16));
     winc = min(winc, dpp<DPP_ROW_BCAST31, 0xc>(INF, winc));  // wave start .. lane
     const int wexcl = dpp<DPP_WAVE_SHR1>(INF, winc);          // wave start .. lane - 1
@@ -683,10 +757,11 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     // round's cost each.  The low 16 bits of xa mark the elements so
     // answered; the answer of the k-th of them is at offset
     // (xa >> (16 + 4k)) & 15 of the previous lane.  They are written with
-    // the staged or patched outputs below.  (Segmented solves: XQ_SEG.)
+    // the staged or patched outputs below.  (Segmented solves: XQ_SEG;
+    // the persistent launch: XQ_SPT.)
     unsigned xa = 0;
 #pragma unroll
-    for (int k = 0; k < (SEG ? XQ_SEG : XQ); k++) {
+    for (int k = 0; k < (SEG ? XQ_SEG : CO ? XQ_SPT : XQ); k++) {
         if (near && qmask) {
             const int i = __builtin_ctz(qmask);
             const int r = last_in_lane<INCL>(sm, tid - 1, tile_value(sm, tid * K + i));
@@ -748,7 +823,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         }
         unsigned xm = xa & 0xFFFFu;
 #pragma unroll
-        for (int k = 0; k < (SEG ? XQ_SEG : XQ); k++)
+        for (int k = 0; k < (SEG ? XQ_SEG : CO ? XQ_SPT : XQ); k++)
             if (xm) {
                 const int i = __builtin_ctz(xm);
                 xm &= xm - 1;
@@ -816,7 +891,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         for (int i = K - 1; i >= 0; i--)
             if (smask & (1u << i)) {
                 if (srank < RS)
-                    to.spine[(size_t)t * RS + srank] = make_int2(tid * K + i, v[i]);
+                    st_i2<CO>(&to.spine[(size_t)t * RS + srank], make_int2(tid * K + i, v[i]));
                 srank++;
             }
     }
@@ -850,7 +925,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         }
         unsigned xm = xa & 0xFFFFu;
 #pragma unroll
-        for (int x = 0; x < (SEG ? XQ_SEG : XQ); x++)
+        for (int x = 0; x < (SEG ? XQ_SEG : CO ? XQ_SPT : XQ); x++)
             if (xm) {
                 const int i = __builtin_ctz(xm), q = i >> 2;
                 xm &= xm - 1;
@@ -882,9 +957,9 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
             total += sm.wopen[w];
             stotal += sm.wspine[w];
         }
-        L.mn0[t] = m;
-        to.qcount[t] = total;
-        to.scount[t] = stotal;
+        st_i<CO>(&L.mn0[t], m);
+        st_i<CO>(&to.qcount[t], total);
+        st_i<CO>(&to.scount[t], stotal);
         if (L1 && L.nlev > 1) atomicMin(&key_at(L, 1, t >> LOG_FAN), make_key(L.epoch, m));
     }
     __syncthreads();  // sm is reused by the next tile
@@ -893,14 +968,14 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
 // Solve tile t (loads included) in the calling workgroup.  With L1 the tile
 // also folds its min into level 1 (an atomic); without, build_level1 makes
 // level 1 from the tile mins after the pass.
-template <bool INCL, bool L1 = true>
+template <bool INCL, bool L1 = true, bool CO = false>
 __device__ __forceinline__ void tile_step(const int* __restrict__ in, int* __restrict__ out,
                                           int n, int t, TileSmem& sm, const TileOut& to,
-                                          const Levels& L) {
+                                          const Levels& L, const int tid = (int)threadIdx.x) {
     int v[K];
-    load_lane(in, n, t, v);
-    const int pl = (threadIdx.x == 0 && t > 0) ? in[t * TILE - 1] : 0;
-    solve_tile<INCL, false, L1>(out, n, t, v, pl, sm, to, L);
+    load_lane(in, n, t, v, tid);
+    const int pl = (tid == 0 && t > 0) ? in[t * TILE - 1] : 0;
+    solve_tile<INCL, false, L1, CO>(out, n, t, v, pl, sm, to, L, SegLane{0, 0u}, tid);
 }
 
 // Segmented tile t: mark the segment starts inside the tile in LDS, give
@@ -970,16 +1045,20 @@ __device__ __forceinline__ void build_level1(const Levels& L) {
     if (lane == 0) key_at(L, 1, g) = make_key(L.epoch, m);
 }
 
-__device__ __forceinline__ void build_upper_levels(const Levels& L) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+// wv: the calling wave's index in its workgroup
+template <bool CO = false>
+__device__ __forceinline__ void build_upper_levels(const Levels& L, int wv) {
+    const int lane = lane_id();
+    const int nwv = blockDim.x >> 6;
     for (int l = 2; l < L.nlev; l++) {
         for (int g = wv; g < L.n[l]; g += nwv) {
             const int idx = g * FAN + lane;
-            const int x = idx < L.n[l - 1] ? key_value(key_at(L, l - 1, idx), L.epoch) : INF;
+            const int x = idx < L.n[l - 1] ? key_value(ld_u64<CO>(&key_at(L, l - 1, idx)), L.epoch)
+                                          : INF;
             const int m = wave_min(x);
-            if (lane == 0) key_at(L, l, g) = make_key(L.epoch, m);
+            if (lane == 0) st_u64<CO>(&key_at(L, l, g), make_key(L.epoch, m));
         }
+        if (CO) wait_own_memory_ops();  // this level's stores complete
         __syncthreads();
     }
 }
@@ -987,19 +1066,20 @@ __device__ __forceinline__ void build_upper_levels(const Levels& L) {
 // Wave-cooperative version of find_tile for one query value v (uniform):
 // every level step is one coalesced 64-entry load and a ballot, so an answer
 // arbitrarily far back costs a few rounds instead of a per-lane walk.
+template <bool CO = false>
 __device__ __forceinline__ int level_entry(const Levels& L, int lvl, int idx) {
-    return lvl == 0 ? L.mn0[idx] : key_value(key_at(L, lvl, idx), L.epoch);
+    return lvl == 0 ? ld_i<CO>(&L.mn0[idx]) : key_value(ld_u64<CO>(&key_at(L, lvl, idx)), L.epoch);
 }
 
-template <bool INCL>
+template <bool INCL, bool CO = false>
 __device__ int find_tile_wave(const Levels& L, int tb, int v) {
-    const int lane = threadIdx.x & (WAVE - 1);
+    const int lane = lane_id();
     int x = tb, lvl = 0, found = -1;  // x: exclusive bound at level lvl
     while (found < 0) {
         if (x <= 0) return -1;  // unreachable when guaranteed
         const int gs = ((x - 1) >> LOG_FAN) << LOG_FAN;
         const int idx = gs + lane;
-        const bool ok = idx < x && qual<INCL>(level_entry(L, lvl, idx), v);
+        const bool ok = idx < x && qual<INCL>(level_entry<CO>(L, lvl, idx), v);
         const u64 b = __ballot(ok);
         if (b) {
             found = gs + 63 - __clzll(b);
@@ -1014,7 +1094,7 @@ __device__ int find_tile_wave(const Levels& L, int tb, int v) {
         lvl--;
         const int gs = found << LOG_FAN;
         const int idx = gs + lane;
-        const bool ok = idx < L.n[lvl] && qual<INCL>(level_entry(L, lvl, idx), v);
+        const bool ok = idx < L.n[lvl] && qual<INCL>(level_entry<CO>(L, lvl, idx), v);
         const u64 b = __ballot(ok);
         found = b ? gs + 63 - __clzll(b) : -1;
         if (found < 0) return -1;  // unreachable: the parent's min qualifies
@@ -1069,7 +1149,7 @@ __device__ __forceinline__ int spine_answer_lds(const int2* sp, int cnt, int v) 
     return pos;
 }
 
-template <bool INCL>
+template <bool INCL, bool CO = false>
 __device__ __forceinline__ int answer_in_tile(const int* __restrict__ in, const TileOut& to,
                                               int s, int v, const int2* sp_lds, int cnt) {
     int pos = -1;
@@ -1083,7 +1163,7 @@ __device__ __forceinline__ int answer_in_tile(const int* __restrict__ in, const 
         for (int h = 0; h < 2 && pos < 0; h++) {
             int4 w[RS / 4];
 #pragma unroll
-            for (int k = 0; k < RS / 4; k++) w[k] = p[h * (RS / 4) + k];
+            for (int k = 0; k < RS / 4; k++) w[k] = ld_i4<CO>(&p[h * (RS / 4) + k]);
 #pragma unroll
             for (int k = RS / 4 - 1; k >= 0; k--) {
                 const int r = h * (RS / 2) + 2 * k;
@@ -1097,12 +1177,12 @@ __device__ __forceinline__ int answer_in_tile(const int* __restrict__ in, const 
     int c = -1;
     if (CPT % 32 == 0) {
         for (int b = s * CPT + CPT - 32; b >= s * CPT && c < 0; b -= 32)
-            c = last_in_32<INCL>(to.cmin, b, v);
+            c = last_in_32<INCL, CO>(to.cmin, b, v);
     } else {
         for (int b = s * CPT + CPT - 16; b >= s * CPT && c < 0; b -= 16) {
             int4 w[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) w[k] = *reinterpret_cast<const int4*>(to.cmin + b + 4 * k);
+            for (int k = 0; k < 4; k++) w[k] = ld_i4<CO>(reinterpret_cast<const int4*>(to.cmin + b + 4 * k));
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 c = qual<INCL>(w[k].x, v) ? b + 4 * k : c;
@@ -1123,7 +1203,7 @@ __device__ __forceinline__ int answer_in_tile(const int* __restrict__ in, const 
 // levels follow from a and b alone, so every level's loads are issued
 // before any is used (one round, not one per level).
 __device__ __forceinline__ int range_min_lane(const Levels& L, int a, int b) {
-    const int lane = threadIdx.x & (WAVE - 1);
+    const int lane = lane_id();
     // raw entries: level 0 as loaded, keys (levels >= 1) converted after
     // every load is issued (0 reads as INF: a key of no run)
     int e0a = INF, e0b = INF;
@@ -1169,7 +1249,7 @@ __device__ __forceinline__ int range_min_lane(const Levels& L, int a, int b) {
 static_assert(CPT <= 2 * WAVE, "the rest of a tile's chunk mins takes two loads per lane");
 __device__ __forceinline__ int seg_prefix_min(const int* __restrict__ in, const TileOut& to,
                                               const Levels& L, int s0, int t) {
-    const int lane = threadIdx.x & (WAVE - 1);
+    const int lane = lane_id();
     const int u = s0 / TILE;
     const bool part = s0 % TILE != 0;
     // all loads first, one wave min at the end
@@ -1185,23 +1265,27 @@ __device__ __forceinline__ int seg_prefix_min(const int* __restrict__ in, const 
     return wave_min(min(min(m0, m1), min(m2, m3)));
 }
 
-template <bool INCL, bool SEG = false>
+// CO: the fix-up runs in the persistent launch that wrote the summaries
+// (tile mins, keys, spines, chunk mins are read coherently; the tile's own
+// open list and outputs were written by this workgroup).
+template <bool INCL, bool SEG = false, bool CO = false>
 __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __restrict__ out,
                                          int n, int t, const Levels& L, const TileOut& to,
                                          int* lds, const SegIn sg = SegIn{nullptr, nullptr, 0, nullptr}) {
+    static_assert(!(SEG && CO), "segmented solves run as separate launches");
     if (t == 0) return;
-    const int lane = threadIdx.x & (WAVE - 1);
+    const int lane = lane_id();
     const int tile_start = t * TILE;
     const int2 tb = SEG ? sg.tbound[t] : make_int2(0, 0);
     const int tw = t - WAVE + lane;  // tile whose min this lane holds
-    const int mw = tw >= 0 ? L.mn0[tw] : INF;
+    const int mw = tw >= 0 ? ld_i<CO>(&L.mn0[tw]) : INF;
     // spines of t-1 (lanes 0..RS-1) and t-2 (lanes RS..2RS-1)
     const int st = (lane < RS) ? t - 1 : t - 2;
     int2 sw = make_int2(-1, INF);
-    if (lane < 2 * RS && st >= 0) sw = to.spine[(size_t)st * RS + (lane & (RS - 1))];
-    const int sc1 = to.scount[t - 1];
-    const int sc2 = (t >= 2) ? to.scount[t - 2] : 0;
-    const int cnt = to.qcount[t];
+    if (lane < 2 * RS && st >= 0) sw = ld_i2<CO>(&to.spine[(size_t)st * RS + (lane & (RS - 1))]);
+    const int sc1 = ld_i<CO>(&to.scount[t - 1]);
+    const int sc2 = (t >= 2) ? ld_i<CO>(&to.scount[t - 2]) : 0;
+    const int cnt = ld_i<CO>(&to.qcount[t]);
     const int v1 = in[tile_start];
     // the list slots are read with the rest of the first batch, before the
     // count is known; slots past the count are ignored below
@@ -1216,8 +1300,8 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     int mw2 = INF;
     if (two) {
         // only groups before the tile's own are used (prefix, deep search)
-        if (lane < (t >> LOG_FAN)) k1raw = key_at(L, 1, lane);
-        if (t > WAVE && gs + lane < t - WAVE) mw2 = L.mn0[gs + lane];
+        if (lane < (t >> LOG_FAN)) k1raw = ld_u64<CO>(&key_at(L, 1, lane));
+        if (t > WAVE && gs + lane < t - WAVE) mw2 = ld_i<CO>(&L.mn0[gs + lane]);
     }
     const int k1 = key_value(k1raw, L.epoch);
     // segmented: only the open elements of the segment that runs into the
@@ -1237,18 +1321,18 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     } else if (L.nlev > 1) {
         const int a1 = t >> LOG_FAN;
         const int b1 = (a1 >> LOG_FAN) << LOG_FAN;
-        if (b1 + lane < a1) p = min(p, key_value(key_at(L, 1, b1 + lane), L.epoch));
+        if (b1 + lane < a1) p = min(p, key_value(ld_u64<CO>(&key_at(L, 1, b1 + lane)), L.epoch));
         if (L.upper) {
 #pragma unroll
             for (int l = 2; l < MAX_LEVELS; l++) {
                 if (l < L.nlev) {
                     const int a = t >> (LOG_FAN * l);
                     const int idx = ((a >> LOG_FAN) << LOG_FAN) + lane;
-                    if (idx < a) p = min(p, key_value(key_at(L, l, idx), L.epoch));
+                    if (idx < a) p = min(p, key_value(ld_u64<CO>(&key_at(L, l, idx)), L.epoch));
                 }
             }
         } else {
-            for (int i = lane; i < b1; i += WAVE) p = min(p, key_value(key_at(L, 1, i), L.epoch));
+            for (int i = lane; i < b1; i += WAVE) p = min(p, key_value(ld_u64<CO>(&key_at(L, 1, i)), L.epoch));
         }
     }
     const int P = SEG ? seg_prefix_min(in, to, L, s0, t) : wave_min(p);
@@ -1304,7 +1388,7 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
             const int vv = __shfl(v, l, WAVE);
             int r = -1;
             if (!two) {
-                r = find_tile_wave<INCL>(L, t - WAVE, vv);
+                r = find_tile_wave<INCL, CO>(L, t - WAVE, vv);
             } else if (t > WAVE) {
                 const u64 b0 = __ballot(gs + lane < t - WAVE && qual<INCL>(mw2, vv));
                 if (b0) {
@@ -1314,7 +1398,7 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
                     if (b1) {
                         const int g = 63 - __clzll(b1);
                         const int idx = (g << LOG_FAN) + lane;
-                        const u64 b2 = __ballot(idx < L.n[0] && qual<INCL>(L.mn0[idx], vv));
+                        const u64 b2 = __ballot(idx < L.n[0] && qual<INCL>(ld_i<CO>(&L.mn0[idx]), vv));
                         r = b2 ? (g << LOG_FAN) + 63 - __clzll(b2) : -1;
                     }
                 }
@@ -1324,9 +1408,9 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
         if (active) {
             int r;
             if (ts < 0) r = -1;  // unreachable: P qualifies, so some tile does
-            else if (ts == t - 1) r = answer_in_tile<INCL>(in, to, ts, v, sp, sc1);
-            else if (ts == t - 2) r = answer_in_tile<INCL>(in, to, ts, v, sp + RS, sc2);
-            else r = answer_in_tile<INCL>(in, to, ts, v, nullptr, to.scount[ts]);
+            else if (ts == t - 1) r = answer_in_tile<INCL, CO>(in, to, ts, v, sp, sc1);
+            else if (ts == t - 2) r = answer_in_tile<INCL, CO>(in, to, ts, v, sp + RS, sc2);
+            else r = answer_in_tile<INCL, CO>(in, to, ts, v, nullptr, ld_i<CO>(&to.scount[ts]));
             if (SEG) r = r >= s0 ? r - s0 : -1;
             out[tile_start + e.x] = r;
         }
@@ -1350,8 +1434,9 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
         ex = (lane == 0) ? rm : min(ex, rm);
         const bool open = valid && !(ex_real && qual<INCL>(ex, xv));
         if (open && qual<INCL>(P, xv)) {
-            const int ts = find_tile<INCL>(L, t, xv);
-            int r = (ts < 0) ? -1 : answer_in_tile<INCL>(in, to, ts, xv, nullptr, to.scount[ts]);
+            const int ts = find_tile<INCL, CO>(L, t, xv);
+            int r = (ts < 0) ? -1 : answer_in_tile<INCL, CO>(in, to, ts, xv, nullptr,
+                                                            ld_i<CO>(&to.scount[ts]));
             if (SEG) r = r >= s0 ? r - s0 : -1;
             out[idx] = r;
         }
@@ -1396,21 +1481,27 @@ __device__ __forceinline__ void grid_barrier_reset(GridBarrier* b) {
 
 // Generation a launch starts from.  The counter persists across launches;
 // it cannot advance before every workgroup has arrived at the first
-// barrier, and each workgroup reads it before arriving, so all workgroups
-// of a launch read the same value.
+// barrier, and each workgroup reads it (at any point) before arriving, so
+// all workgroups of a launch read the same value.
 __device__ __forceinline__ unsigned grid_barrier_start(GridBarrier* b) {
     return __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Barrier number `j` (0, 1, ...) of the launch.  Returns false (uniformly
+// Barrier number `j` (0, 1, ...) of a launch whose cross-workgroup data is
+// written and read with device-coherent (CO) accesses, so no fence is
+// needed: every lane waits for its own memory operations to complete (the
+// "memory" clobber also keeps the compiler from moving any of them past the
+// wait), then the workgroup arrives; after the generation changes, the CO
+// loads read the other workgroups' words from the coherent level.  `leader`
+// is true for exactly one lane of the workgroup.  Returns false (uniformly
 // per workgroup) if the barrier timed out.
-__device__ __forceinline__ bool grid_barrier(GridBarrier* b, unsigned nblocks,
-                                             unsigned& gen, unsigned& j) {
+__device__ __forceinline__ bool grid_barrier_co(GridBarrier* b, unsigned nblocks,
+                                                unsigned& gen, unsigned& j, bool leader) {
     __shared__ int s_ok;
+    wait_own_memory_ops();
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (leader) {
         int ok = 1;
-        __threadfence();  // publish this block's writes (agent scope)
         const unsigned g = blockIdx.x % BAR_SUB;
         const unsigned nsub = nblocks < (unsigned)BAR_SUB ? nblocks : (unsigned)BAR_SUB;
         const unsigned gsz = nblocks / BAR_SUB + (g < nblocks % BAR_SUB ? 1u : 0u);
@@ -1419,19 +1510,18 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier* b, unsigned nblocks,
             atomicAdd(&b->root, 1u) == (j + 1) * nsub - 1)
             release = true;
         if (release) {
-            __hip_atomic_store(&b->gen, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&b->gen, gen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             unsigned spins = 0;
-            while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 if (++spins > BARRIER_SPIN_LIMIT) {
                     __hip_atomic_store(&b->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ok = 0;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(4);
+                __builtin_amdgcn_s_sleep(2);
             }
         }
-        __threadfence();  // acquire the other blocks' writes
         s_ok = ok;
     }
     __syncthreads();

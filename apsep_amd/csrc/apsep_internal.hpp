@@ -59,6 +59,20 @@ struct apsep_batch_plan {
 
 namespace apsep {
 
+// The HIP error behind the calling thread's last failing status: recorded
+// where the library clears HIP's sticky error (apsep_last_error reports it).
+void record_hip_error(hipError_t e, const char* func, int line);
+// Reads and clears HIP's last error; records it when it is one.
+inline hipError_t take_hip_error(const char* func, int line) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) record_hip_error(e, func, line);
+    return e;
+}
+#define APSEP_CLEAR_HIP_ERROR() ((void)::apsep::take_hip_error(__func__, __LINE__))
+// true when the launches just issued failed (the error is recorded)
+#define APSEP_LAUNCH_FAILED() (::apsep::take_hip_error(__func__, __LINE__) != hipSuccess)
+
+
 int level_sizes(int nt, int* sizes);
 int64_t single_scratch_bytes(int64_t n_cap, apsep_algo algo, int tile);
 apsep_status run_lookback(apsep_scratch* s, const int32_t* in, int32_t* out, int n, int incl,
@@ -83,9 +97,11 @@ apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs
 int seg_tile_elems();
 }  // namespace seg512
 
-// One full solve, then `reps` back-to-back launches of each of its kernels
-// (each launch is idempotent), timed per kernel batch with HIP events.
-apsep_status single_time_kernels(apsep_scratch* s, const int32_t* in, int32_t* out, int n,
-                                 int incl, int reps, hipStream_t st, double ms[3]);
+// `reps` back-to-back launches of each kernel of the solve (each launch is
+// idempotent), timed per kernel batch with HIP events; launch r solves array
+// r % m (its own scratch, input and output).
+apsep_status single_time_kernels(apsep_scratch* const* ss, const int32_t* const* ins,
+                                 int32_t* const* outs, int m, int n, int incl, int reps,
+                                 hipStream_t st, double ms[3]);
 
 }  // namespace apsep

@@ -313,7 +313,7 @@ static apsep_status run_lookback_t(apsep_scratch* s, const int* in, int* out, in
     }
     hipLaunchKernelGGL(k_lookback_begin, dim3(1), dim3(1), 0, st, s->lb_ctl);
     hipLaunchKernelGGL(k_lookback<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L, V);
-    return hipGetLastError() == hipSuccess ? APSEP_OK : APSEP_ERR_LAUNCH;
+    return APSEP_LAUNCH_FAILED() ? APSEP_ERR_LAUNCH : APSEP_OK;
 }
 
 apsep_status run_lookback(apsep_scratch* s, const int32_t* in, int32_t* out, int n, int incl,

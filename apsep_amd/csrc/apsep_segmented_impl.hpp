@@ -139,7 +139,7 @@ static apsep_status seg_launch(apsep_batch_plan* p, const int* in, const long lo
         hipLaunchKernelGGL(k_fix_pass_seg<INCL>, dim3((nt + tpw - 1) / tpw), dim3(SEG_FIX_WG), 0,
                            st, in, out, n, L, to, sg);
     }
-    return hipGetLastError() == hipSuccess ? APSEP_OK : APSEP_ERR_LAUNCH;
+    return APSEP_LAUNCH_FAILED() ? APSEP_ERR_LAUNCH : APSEP_OK;
 }
 
 apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs,

@@ -145,11 +145,11 @@ static apsep_status retire(apsep_server* s, bool wait) {
         Inflight& f = s->inf[s->inf_head];
         hipError_t e = wait ? hipEventSynchronize(f.done) : hipEventQuery(f.done);
         if (e == hipErrorNotReady) {
-            (void)hipGetLastError();
+            APSEP_CLEAR_HIP_ERROR();
             return APSEP_OK;
         }
         if (e != hipSuccess) {
-            (void)hipGetLastError();
+            APSEP_CLEAR_HIP_ERROR();
             return APSEP_ERR_LAUNCH;
         }
         const double t = now_us();
@@ -200,7 +200,7 @@ static apsep_status launch_batch(apsep_server* s) {
     }
     if (hipMemcpyAsync(f.d_off, f.h_off, (size_t)(3 * m + 1) * 8, hipMemcpyHostToDevice,
                        s->stream) != hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_LAUNCH;
     }
     const long long nblk = (total + GATHER - 1) / GATHER;
@@ -208,12 +208,12 @@ static apsep_status launch_batch(apsep_server* s) {
                        (const long long*)f.d_off, (const unsigned long long*)(f.d_off + m + 1),
                        (const long long*)(f.d_off + 2 * m + 1), (int)m, s->ring_in + start,
                        (long long)total);
-    if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
+    if (APSEP_LAUNCH_FAILED()) return APSEP_ERR_LAUNCH;
     apsep_status st = apsep_run_segmented(s->plan, s->ring_in + start, f.d_off,
                                           s->ring_out + start, m, total, s->stream);
     if (st != APSEP_OK) return st;
     if (hipEventRecord(f.done, s->stream) != hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_LAUNCH;
     }
     f.busy = true;
@@ -241,12 +241,12 @@ apsep_status apsep_server_create(int64_t pool_bytes, int64_t max_batch, int32_t 
     s->max_segs = max_batch_jobs;
     if (hipGetDevice(&s->device) != hipSuccess ||
         hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         destroy_server(s);
         return APSEP_ERR_DEVICE;
     }
     if (hipMalloc((void**)&s->ring_in, (size_t)s->cap * 8) != hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         s->ring_in = nullptr;
         destroy_server(s);
         return APSEP_ERR_ALLOC;
@@ -263,7 +263,7 @@ apsep_status apsep_server_create(int64_t pool_bytes, int64_t max_batch, int32_t 
         if (hipEventCreateWithFlags(&f.done, hipEventDisableTiming) != hipSuccess ||
             hipHostMalloc((void**)&f.h_off, (size_t)(3 * max_batch_jobs + 1) * 8, 0) != hipSuccess ||
             hipMalloc((void**)&f.d_off, (size_t)(3 * max_batch_jobs + 1) * 8) != hipSuccess) {
-            (void)hipGetLastError();
+            APSEP_CLEAR_HIP_ERROR();
             destroy_server(s);
             return APSEP_ERR_ALLOC;
         }

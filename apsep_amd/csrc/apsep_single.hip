@@ -50,7 +50,9 @@ __global__ __launch_bounds__(BAR_SUB) void k_spt_begin(GridBarrier* bar) {
 }
 
 // WSTL pass 2 (only when level 1 exceeds UPPER_MIN entries): levels >= 2.
-__global__ __launch_bounds__(256) void k_upper_levels(Levels L) { build_upper_levels(L); }
+__global__ __launch_bounds__(256) void k_upper_levels(Levels L) {
+    build_upper_levels(L, (int)(threadIdx.x >> 6));
+}
 
 // WSTL pass 3: one wave per tile.
 // (256-lane workgroups of four tiles, held to 64 VGPRs so that 8 waves fit
@@ -69,31 +71,49 @@ __global__ __launch_bounds__(FIX_WG) __attribute__((amdgpu_waves_per_eu(8))) voi
 }
 
 // SPT: one persistent launch of fully co-resident workgroups (launched
-// cooperatively): tile phase, grid barrier, [upper levels by workgroup 0,
-// grid barrier,] fix-up phase, all grid-strided.  The occupancy attribute
-// guarantees SPT_WAVES_PER_EU waves per SIMD (that many four-wave
-// workgroups per CU) and the host never launches more per CU, so the grid is
-// co-resident whatever register budget the compiler picks.
+// cooperatively), three per CU: the tile phase over tiles blockIdx.x,
+// blockIdx.x + grid, ..., a grid barrier, [upper levels by workgroup 0, a
+// grid barrier,] then the fix-up of the same tiles, one wave per tile.  A
+// tile is fixed by the workgroup that solved it, so its outputs and open
+// list stay in one XCD's L2; the summaries other workgroups read (tile
+// mins, chunk mins, spines, counts, upper keys) are written and read with
+// device-coherent accesses, and the barrier needs no fence.  The occupancy
+// attribute guarantees SPT_WAVES_PER_EU waves per SIMD and the host never
+// launches more workgroups per CU, so the grid is co-resident whatever
+// register budget the compiler picks.  The fix-up's LDS reuses the tile's.
 template <bool INCL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SPT_WAVES_PER_EU)))
 void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
            TileOut to, Levels L, GridBarrier* bar) {
     __shared__ TileSmem sm;
-    __shared__ int lds[NWAVE][FIX_LDS_INTS];
-    unsigned gen = grid_barrier_start(bar), j = 0;
+    static_assert(NWAVE * FIX_LDS_INTS * sizeof(int) <= sizeof(TileSmem), "fix-up LDS");
+    // Nothing per-lane stays live across the tile loop (it would be spilled
+    // to memory): the wave index is kept in a scalar, and each tile computes
+    // the lane's index afresh inside a volatile asm, so the compiler cannot
+    // hoist the lane-derived addresses out of the loop.
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int nt = L.nt;
-    for (int t = (int)blockIdx.x; t < nt; t += (int)gridDim.x)
-        tile_step<INCL>(in, out, n, t, sm, to, L);
-    if (nt <= 1) return;  // uniform across the grid
-    if (!grid_barrier(bar, gridDim.x, gen, j)) return;
-    if (L.upper) {
-        if (blockIdx.x == 0) build_upper_levels(L);
-        if (!grid_barrier(bar, gridDim.x, gen, j)) return;
+    const int grid = (int)gridDim.x;
+    for (int t = (int)blockIdx.x; t < nt; t += grid) {
+        int lane;
+        asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0"
+                     : "=v"(lane));
+        tile_step<INCL, true, true>(in, out, n, t, sm, to, L, wave * WAVE + lane);
     }
-    const int gw = (int)(blockIdx.x * NWAVE + (threadIdx.x >> 6));
-    const int nw = (int)(gridDim.x * NWAVE);
-    for (int t = gw; t < nt; t += nw)
-        fix_tile<INCL>(in, out, n, t, L, to, lds[threadIdx.x >> 6]);
+    if (nt <= 1) return;  // uniform across the grid
+    const bool leader = wave == 0 && lane_id() == 0;
+    unsigned gen = grid_barrier_start(bar), j = 0;  // before this workgroup arrives
+    if (!grid_barrier_co(bar, gridDim.x, gen, j, leader)) return;
+    if (L.upper) {
+        if (blockIdx.x == 0) build_upper_levels<true>(L, wave);
+        if (!grid_barrier_co(bar, gridDim.x, gen, j, leader)) return;
+    }
+    int* lds = reinterpret_cast<int*>(&sm) + wave * FIX_LDS_INTS;
+    for (int k = wave;; k += NWAVE) {
+        const long long t = (long long)blockIdx.x + (long long)k * grid;
+        if (t >= nt) break;  // uniform per wave
+        fix_tile<INCL, false, true>(in, out, n, (int)t, L, to, lds);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -218,19 +238,19 @@ apsep_status single_scratch_init(apsep_scratch* s) {
     s->bytes = single_scratch_bytes(s->n_cap, s->algo, s->tile);
     if (hipMalloc(&s->base, (size_t)s->bytes) != hipSuccess) {
         s->base = nullptr;
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_ALLOC;
     }
     if (!carve_arrays(s)) return APSEP_ERR_ALLOC;  // sizing bug guard
     s->epoch = FIRST_EAGER_EPOCH;
     if (hipMemset(s->base, 0, (size_t)s->bytes) != hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_LAUNCH;
     }
     // keys start above every real key (a fresh scratch holds no run)
     for (int l = 1; l < nl; l++)
         if (hipMemset(s->key[l], 0xFF, key_slot(l, round4(sz[l]) + 4) * 8) != hipSuccess) {
-            (void)hipGetLastError();
+            APSEP_CLEAR_HIP_ERROR();
             return APSEP_ERR_LAUNCH;
         }
 
@@ -245,12 +265,12 @@ apsep_status single_scratch_init(apsep_scratch* s) {
         const int pinned = SPT_WAVES_PER_EU * 4 / NWAVE;
         if (per_cu > pinned) per_cu = pinned;
         if (e != hipSuccess || per_cu <= 0) {
-            (void)hipGetLastError();
+            APSEP_CLEAR_HIP_ERROR();
             return APSEP_ERR_UNSUPPORTED;
         }
         int coop = 0;
         if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, s->device) != hipSuccess || !coop) {
-            (void)hipGetLastError();
+            APSEP_CLEAR_HIP_ERROR();
             return APSEP_ERR_UNSUPPORTED;
         }
         s->coop_grid = per_cu * s->num_cus;
@@ -263,7 +283,7 @@ apsep_status single_scratch_init(apsep_scratch* s) {
 static int capture_state(hipStream_t st) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return -1;
     }
     return cs == hipStreamCaptureStatusNone ? 0 : cs == hipStreamCaptureStatusActive ? 1 : -1;
@@ -286,7 +306,7 @@ static apsep_status fill_levels(apsep_scratch* s, int nt, Levels& L, hipStream_t
         if (L.nlev > 1) {
             const int blocks = (sz[1] + 255) / 256;
             hipLaunchKernelGGL(k_reset_level1, dim3(blocks), dim3(256), 0, st, s->key[1], sz[1]);
-            if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
+            if (APSEP_LAUNCH_FAILED()) return APSEP_ERR_LAUNCH;
         }
         L.epoch = CAPTURE_EPOCH;
         return APSEP_OK;
@@ -316,14 +336,14 @@ static apsep_status run_wstl(apsep_scratch* s, const int* in, int* out, int n,
     if (fill_levels(s, nt, L, st, cap) != APSEP_OK) return APSEP_ERR_LAUNCH;
     TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
     hipLaunchKernelGGL(k_tile_pass<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L);
-    if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
+    if (APSEP_LAUNCH_FAILED()) return APSEP_ERR_LAUNCH;
     if (nt > 1 && L.upper)
         hipLaunchKernelGGL(k_upper_levels, dim3(1), dim3(256), 0, st, L);
     if (nt > 1) {
         constexpr int tpw = FIX_WG / WAVE;
         hipLaunchKernelGGL(k_fix_pass<INCL>, dim3((nt + tpw - 1) / tpw), dim3(FIX_WG), 0, st,
                            in, out, n, L, to);
-        if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
+        if (APSEP_LAUNCH_FAILED()) return APSEP_ERR_LAUNCH;
     }
     return APSEP_OK;
 }
@@ -341,63 +361,77 @@ static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
     if (grid < 1) grid = 1;
     GridBarrier* bar = s->bar;
     hipLaunchKernelGGL(k_spt_begin, dim3(1), dim3(BAR_SUB), 0, st, bar);
-    if (hipGetLastError() != hipSuccess) return APSEP_ERR_LAUNCH;
+    if (APSEP_LAUNCH_FAILED()) return APSEP_ERR_LAUNCH;
     void* args[] = {(void*)&in, (void*)&out, (void*)&n, (void*)&to, (void*)&L, (void*)&bar};
     hipError_t e = hipLaunchCooperativeKernel((const void*)k_spt<INCL>, dim3(grid), dim3(NT),
                                               args, 0, st);
     if (e != hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_LAUNCH;
     }
     return APSEP_OK;
 }
 
+// Launch r of each kernel solves array r % m (its own scratch, input and
+// output); with m = 1 the same array every time.
 template <bool INCL>
-static apsep_status time_wstl(apsep_scratch* s, const int* in, int* out, int n, int reps,
-                              hipStream_t st, hipEvent_t* ev) {
+static apsep_status time_wstl(apsep_scratch* const* ss, const int* const* ins, int* const* outs,
+                              int m, int n, int reps, hipStream_t st, hipEvent_t* ev) {
     const int nt = (int)(((long long)n + TILE - 1) / TILE);  // (n + TILE - 1 overflows near 2^31)
-    Levels L;
-    if (fill_levels(s, nt, L, st) != APSEP_OK) return APSEP_ERR_LAUNCH;
-    TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
+    constexpr int MAXM = 8;
+    Levels L[MAXM];
+    TileOut to[MAXM];
+    for (int k = 0; k < m; k++) {
+        if (fill_levels(ss[k], nt, L[k], st) != APSEP_OK) return APSEP_ERR_LAUNCH;
+        to[k] = TileOut{ss[k]->cmin, ss[k]->qcount, ss[k]->qlist, ss[k]->scount, ss[k]->spine};
+    }
     constexpr int tpw = FIX_WG / WAVE;
     // re-running a kernel on the same epoch rewrites the same values: the
     // level-1 atomicMin sees the same keys, the fix-up the same summaries
     (void)hipEventRecord(ev[0], st);
-    for (int r = 0; r < reps; r++)
-        hipLaunchKernelGGL(k_tile_pass<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L);
+    for (int r = 0; r < reps; r++) {
+        const int k = r % m;
+        hipLaunchKernelGGL(k_tile_pass<INCL>, dim3(nt), dim3(NT), 0, st, ins[k], outs[k], n, to[k],
+                           L[k]);
+    }
     (void)hipEventRecord(ev[1], st);
-    if (nt > 1 && L.upper)
+    if (nt > 1 && L[0].upper)
         for (int r = 0; r < reps; r++)
-            hipLaunchKernelGGL(k_upper_levels, dim3(1), dim3(256), 0, st, L);
+            hipLaunchKernelGGL(k_upper_levels, dim3(1), dim3(256), 0, st, L[r % m]);
     (void)hipEventRecord(ev[2], st);
     if (nt > 1)
-        for (int r = 0; r < reps; r++)
+        for (int r = 0; r < reps; r++) {
+            const int k = r % m;
             hipLaunchKernelGGL(k_fix_pass<INCL>, dim3((nt + tpw - 1) / tpw), dim3(FIX_WG), 0,
-                               st, in, out, n, L, to);
+                               st, ins[k], outs[k], n, L[k], to[k]);
+        }
     (void)hipEventRecord(ev[3], st);
-    return hipGetLastError() == hipSuccess ? APSEP_OK : APSEP_ERR_LAUNCH;
+    return APSEP_LAUNCH_FAILED() ? APSEP_ERR_LAUNCH : APSEP_OK;
 }
 
-apsep_status single_time_kernels(apsep_scratch* s, const int32_t* in, int32_t* out, int n,
-                                 int incl, int reps, hipStream_t st, double ms[3]) {
+apsep_status single_time_kernels(apsep_scratch* const* ss, const int32_t* const* ins,
+                                 int32_t* const* outs, int m, int n, int incl, int reps,
+                                 hipStream_t st, double ms[3]) {
+    if (m < 1 || m > 8) return APSEP_ERR_INVALID_ARG;
     if (capture_state(st) != 0) return APSEP_ERR_UNSUPPORTED;  // it synchronises
     hipEvent_t ev[4];
     int made = 0;
     apsep_status r = APSEP_OK;
     for (; made < 4; made++)
         if (hipEventCreate(&ev[made]) != hipSuccess) {
-            (void)hipGetLastError();
+            APSEP_CLEAR_HIP_ERROR();
             r = APSEP_ERR_DEVICE;
             break;
         }
     if (r == APSEP_OK) {
-        if (s->algo == APSEP_ALGO_WSTL) {
-            r = incl ? time_wstl<true>(s, in, out, n, reps, st, ev)
-                     : time_wstl<false>(s, in, out, n, reps, st, ev);
+        if (ss[0]->algo == APSEP_ALGO_WSTL) {
+            r = incl ? time_wstl<true>(ss, ins, outs, m, n, reps, st, ev)
+                     : time_wstl<false>(ss, ins, outs, m, n, reps, st, ev);
         } else {
             // the persistent kernel is the whole solve
             (void)hipEventRecord(ev[0], st);
-            for (int k = 0; k < reps && r == APSEP_OK; k++) r = single_run(s, in, out, n, incl, st);
+            for (int k = 0; k < reps && r == APSEP_OK; k++)
+                r = single_run(ss[k % m], ins[k % m], outs[k % m], n, incl, st);
             (void)hipEventRecord(ev[1], st);
             (void)hipEventRecord(ev[2], st);
             (void)hipEventRecord(ev[3], st);
@@ -410,7 +444,7 @@ apsep_status single_time_kernels(apsep_scratch* s, const int32_t* in, int32_t* o
             if (hipEventElapsedTime(&t, ev[k], ev[k + 1]) != hipSuccess) r = APSEP_ERR_LAUNCH;
             ms[k] = t;
         }
-        if (r != APSEP_OK) (void)hipGetLastError();
+        if (r != APSEP_OK) APSEP_CLEAR_HIP_ERROR();
     }
     for (int k = 0; k < made; k++) (void)hipEventDestroy(ev[k]);
     return r;
@@ -459,7 +493,7 @@ static apsep_status noalloc_setup(int dev) {
     if (g_noalloc_ready[dev]) return APSEP_OK;
     void* ptr = nullptr;
     if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_DEVICE;
     }
     apsep_scratch& s = g_noalloc[dev];
@@ -480,7 +514,7 @@ static apsep_status noalloc_setup(int dev) {
 apsep_status noalloc_run(const int32_t* in, int32_t* out, int n, int incl, hipStream_t st) {
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_DEVICE;
     }
     if (dev >= NOALLOC_MAX_DEVICES) return APSEP_ERR_DEVICE;
@@ -495,7 +529,7 @@ apsep_status single_error(apsep_scratch* s, int* code) {
     if (s->lb_ctl) {
         int lerr = 0;
         if (hipMemcpy(&lerr, &s->lb_ctl->err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
-            (void)hipGetLastError();
+            APSEP_CLEAR_HIP_ERROR();
             return APSEP_ERR_LAUNCH;
         }
         if (lerr) {
@@ -506,14 +540,14 @@ apsep_status single_error(apsep_scratch* s, int* code) {
     if (!s->bar) return APSEP_OK;
     int err = 0;
     if (hipMemcpy(&err, &s->bar->err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
-        (void)hipGetLastError();
+        APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_LAUNCH;
     }
     if (err) {
         // cleared once read (the counters are zeroed before every launch)
         const int zero = 0;
         if (hipMemcpy(&s->bar->err, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
-            (void)hipGetLastError();
+            APSEP_CLEAR_HIP_ERROR();
             return APSEP_ERR_LAUNCH;
         }
     }

@@ -58,6 +58,14 @@ const char* apsep_status_string(apsep_status s);
 /* Library version as "major.minor.patch". */
 const char* apsep_version(void);
 
+/* What lies behind the calling thread's last failing status: the HIP error
+ * (name, text, library function and line) or the device-side error; "none"
+ * when nothing has failed.  No counterpart in the reference, whose gpuAssert
+ * prints the CUDA error text and exits (src/apsep.cuh:39-47); the apsep.hpp
+ * shims print it the same way.  Static per-thread storage, overwritten by
+ * the next failure. */
+const char* apsep_last_error(void);
+
 /* Replaces allocWSTLScratch (src/apsep.cuh:2520-2537) and allocSPTScratch
  * (src/apsep.cuh:3013-3040): device scratch sized for up to n_cap elements,
  * bound to the current device.  n_cap may be 0 (then only n = 0 runs). */
@@ -150,6 +158,17 @@ typedef struct apsep_kernel_times {
 apsep_status apsep_time_kernels(apsep_scratch* s, const int32_t* d_in, int32_t* d_out,
                                 int64_t n, int incl, int reps, void* stream,
                                 apsep_kernel_times* out);
+
+/* apsep_time_kernels over `count` (1..8) arrays of n keys in turn: launch r
+ * of each kernel solves array r % count with its own scratch (same
+ * algorithm, same device), input and output, so consecutive launches stream
+ * different data through the caches, as consecutive solves of different
+ * arrays do.  Each d_out[k] holds its answer afterwards.  No counterpart in
+ * the reference. */
+apsep_status apsep_time_kernels_rotating(apsep_scratch* const* s, const int32_t* const* d_in,
+                                         int32_t* const* d_out, int32_t count, int64_t n,
+                                         int incl, int reps, void* stream,
+                                         apsep_kernel_times* out);
 
 /* ---- Segmented (CSR) batches ---------------------------------------------
  * No counterpart in the reference, whose callers solve many arrays with one

@@ -16,8 +16,8 @@
 //     own wave64 tile shape (512 lanes x 16 elements).
 //   * run* launch on the default stream, asynchronously, like the reference;
 //     an optional trailing stream argument (hipStream_t as void*) is added.
-//   * Errors print "APSEP error <file>:<line>: <text>" and exit, as the
-//     reference's gpuAssert does (src/apsep.cuh:39-47).  The C ABI underneath
+//   * Errors print "APSEP error <file>:<line>: <text> (<HIP error>)" and
+//     exit, as the reference's gpuAssert does (src/apsep.cuh:39-47).  The C ABI underneath
 //     (apsep.h) returns status codes instead.
 #ifndef APSEP_APSEP_HPP
 #define APSEP_APSEP_HPP
@@ -33,8 +33,8 @@
 namespace apsep_shim {
 inline void check(apsep_status st, const char* file, int line) {
     if (st != APSEP_OK) {
-        std::fprintf(stderr, "APSEP error %s:%d: %s\n", file, line,
-                     apsep_status_string(st));
+        std::fprintf(stderr, "APSEP error %s:%d: %s (%s)\n", file, line,
+                     apsep_status_string(st), apsep_last_error());
         std::exit(EXIT_FAILURE);
     }
 }
