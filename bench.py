@@ -1,25 +1,40 @@
-"""Benchmark: useful GB/s of the plain single-array APSEP solve on MI355X.
+"""Benchmark: useful GB/s of the plain single-array APSEP solve on MI355X,
+plus the batched and served forms.
 
-Workload (the headline configuration): N = 2^25 int32 keys in three classes
--- uniform random in [0, 2^31), strictly descending (a[i] = N - i), strictly
-ascending (a[i] = i).  One step = one solve of each of the three arrays, all
-resident in HBM before timing starts.  Useful bytes per solve = 2 * N * 4
-(read the keys once, write the answers once).
+Headline workload: N = 2^25 int32 keys in three classes -- uniform random
+in [0, 2^31), strictly descending (a[i] = N - i), strictly ascending
+(a[i] = i).  One step = one solve of each of the three arrays (each with its
+own input and output buffer), all resident in HBM before timing starts.
+Useful bytes per solve = 2 * N * 4 (read the keys once, write the answers
+once).
 
 Printed JSON line (rank 0, one line):
   value        aggregate useful GB/s over the three solves of a step
-  roofline     the dominant kernel (the tile pass): achieved = 8 N / (its
-               mean device time: after the timed steps, each kernel of the
-               solve is launched `steps` times back to back per class with a
-               HIP event pair around each batch on the same stream), peak =
-               8000 GB/s HBM;
-               traffic = its HBM bytes per launch from rocprofv3
-               FETCH_SIZE/WRITE_SIZE runs scaled by a plain copy's counts
-               (profiles/traffic.json; null when not measured)
+  settle_s     untimed solves run before the warmup steps so that the GPU
+               reaches its sustained clock (the first milliseconds after an
+               idle GPU run slower)
+  roofline     the dominant kernel (the tile pass): achieved = 8 N / its
+               mean device time, measured as in the timed steps' cache
+               regime -- each kernel of the solve launched back to back over
+               the three classes in turn, each class with its own scratch,
+               input and output (768 MB, three times the last-level cache),
+               one HIP event pair around each kernel's batch on the stream
+               the kernels run on; peak = 8000 GB/s HBM; traffic = its HBM
+               bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+               scaled by a plain copy's counts (profiles/traffic.json; null
+               when not measured).  warm_* beside it: the same kernels
+               re-run on one array (its data partly cache-resident)
   cpu_baseline the oracle's monotone stack (1 thread) on a bounded sample of
                the same inputs, timed on this host
-  per_class_*  each class alone: `steps` solves back to back between one
-               event pair, after the timed region
+  per_class_*  each class alone: solves back to back between one event pair
+  batch        the batched configuration (8192 segments, log-uniform lengths
+               64..2^20, ~8.7e8 keys): arrays/s and useful GB/s of one
+               segmented solve, against one apsep_run and one apsep_launch
+               per array on the same arrays
+  serve        the served configuration (Poisson arrivals, lengths
+               log-uniform 2^10..2^24, classes 60/20/20, 64 GB pool) at a few
+               offered rates: jobs/s and p50/p99 latency, against one
+               apsep_run and one apsep_launch per job on the same job mix
 Timing: HIP events on the stream the kernels are launched on (torch's current
 stream, passed to the C ABI); the timed region holds the `steps` steps and no
 other event.
@@ -53,12 +68,27 @@ def make_inputs(torch, n: int, seed: int = 2024):
     }
 
 
-def time_solves(torch, scratch, xs, outs, steps, warmup, stream):
+def settle(torch, scratch, xs, outs, stream, seconds):
+    """Untimed solves for `seconds` of wall time (at least one round)."""
+    names = list(xs)
+    t0 = time.perf_counter()
+    rounds = 0
+    while rounds == 0 or time.perf_counter() - t0 < seconds:
+        for _ in range(10):
+            for k in names:
+                scratch.run(xs[k], outs[k], stream=stream)
+        torch.cuda.synchronize()
+        rounds += 10
+    return rounds
+
+
+def time_solves(torch, apsep_amd, scratch, xs, outs, steps, warmup, stream, per_reps):
     """Timed region: `steps` steps of one solve per class, back to back on
     `stream` with nothing between the solves, one HIP event pair around the
-    whole region.  Then, outside it, the per-class breakdown: `steps` solves
-    of one class back to back between one event pair, and each kernel of the
-    solve launched `steps` times back to back per class."""
+    whole region.  Then, outside it: the per-class breakdown (`per_reps`
+    solves of one class back to back between one event pair), each kernel
+    of the solve launched over the three classes in turn (one scratch per
+    class), and the same kernels re-run on one class at a time (warm)."""
     names = list(xs)
     for _ in range(warmup):
         for k in names:
@@ -78,31 +108,27 @@ def time_solves(torch, scratch, xs, outs, steps, warmup, stream):
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         a.record(stream)
-        for _ in range(steps):
+        for _ in range(per_reps):
             scratch.run(xs[k], outs[k], stream=stream)
         b.record(stream)
         torch.cuda.synchronize()
-        per[k] = a.elapsed_time(b) / steps
+        per[k] = a.elapsed_time(b) / per_reps
     scratch.check()
-    kt = {k: scratch.time_kernels(xs[k], outs[k], steps, stream=stream) for k in names}
+    n = xs[names[0]].numel()
+    algo = {v: k for k, v in apsep_amd.ALGOS.items()}[scratch.algo]
+    rot_s = [apsep_amd.Scratch(algo, n) for _ in names]
+    try:
+        reps = per_reps * len(names)
+        rot = apsep_amd.time_kernels_rotating(rot_s, [xs[k] for k in names],
+                                              [outs[k] for k in names], reps, stream=stream)
+        for r in rot_s:
+            r.check()
+    finally:
+        for r in rot_s:
+            r.close()
+    warm = {k: scratch.time_kernels(xs[k], outs[k], per_reps, stream=stream) for k in names}
     scratch.check()
-    return per, t_all0.elapsed_time(t_all1), kt
-
-
-def copy_gbs(torch, n, stream, reps=20):
-    src = torch.empty(n, dtype=torch.int32, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    a = torch.cuda.Event(enable_timing=True)
-    b = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    a.record(stream)
-    for _ in range(reps):
-        dst.copy_(src)
-    b.record(stream)
-    torch.cuda.synchronize()
-    return 8.0 * n * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+    return per, t_all0.elapsed_time(t_all1), rot, warm
 
 
 def cpu_baseline(torch, xs, outs, budget_s: float):
@@ -161,59 +187,42 @@ def batch_inputs(torch, nseg: int, lo: int, hi: int, seed: int):
     return x.contiguous(), off, lengths, cls
 
 
-def bench_batch(args, torch, apsep_amd):
-    """Segmented batch: 8192 segments with log-uniform lengths in [64, 2^20]
-    (about 8.9e8 keys), one class per segment.  Reports the batched solve
-    (eager and hipGraph replay) against one solve per array (the reference's
-    calling pattern: a persistent scratch and apsep_run per array, and
-    apsep_launch -- allocate, solve, synchronise, free -- per array)."""
-    import numpy as np
-    stream = torch.cuda.current_stream()
-    x, off, lengths, cls = batch_inputs(torch, args.nseg, 64, 1 << 20, 2024)
-    total, nseg = int(off[-1]), args.nseg
-    offs = torch.from_numpy(off).cuda()
-    out = torch.empty(total, dtype=torch.int32, device="cuda")
+def _timed(torch, stream, fn, reps):
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-
-    def timed(fn, reps):
-        torch.cuda.synchronize()
-        ev0.record(stream)
-        for _ in range(reps):
-            fn()
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        return ev0.elapsed_time(ev1) / reps
-
-    res = {}
-    stream = torch.cuda.Stream()  # graphs need a capturable (non-null) stream
-    with apsep_amd.BatchPlan(total, nseg) as p:
-        run = lambda: p.run(x, offs, out, total=total, stream=stream)
-        with torch.cuda.stream(stream):
-            # both tile shapes x direct/graph; the plan keeps the fastest
-            tuned = p.tune(x, offs, out, total=total, stream=stream)
-            res["tuned_mode"] = "graph" if tuned[1] < tuned[0] else "eager"
-            res["tile_lanes"] = p.tile_lanes
-            p.use_graph(False)
-            for _ in range(args.warmup):
-                run()
-            res["eager_ms"] = timed(run, args.steps)
-            p.use_graph(True)
-            run()
-            res["graph_ms"] = timed(run, args.steps)
     torch.cuda.synchronize()
-    got = out.cpu().numpy()
+    ev0.record(stream)
+    for _ in range(reps):
+        fn()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / reps
 
-    # one solve per array, each array in its own 16-byte aligned slot
+
+def _packed(torch, x, off, lengths):
+    """Every array of a CSR batch in its own 16-byte aligned slot (as a
+    caller with one buffer per array has them); returns (keys, answers,
+    slot starts)."""
+    import numpy as np
+    nseg = lengths.shape[0]
     pstart = np.zeros(nseg + 1, dtype=np.int64)
     np.cumsum((lengths + 3) // 4 * 4, out=pstart[1:])
-    ptotal = int(pstart[-1])
     seg = torch.repeat_interleave(torch.arange(nseg, device="cuda"),
                                   torch.from_numpy(lengths).cuda())
-    dst = torch.arange(total, device="cuda") - offs[:-1][seg] + torch.from_numpy(pstart[:-1]).cuda()[seg]
-    xp = torch.zeros(ptotal, dtype=torch.int32, device="cuda")
+    offs = torch.from_numpy(off).cuda()
+    dst = torch.arange(x.numel(), device="cuda") - offs[:-1][seg] + \
+        torch.from_numpy(pstart[:-1]).cuda()[seg]
+    xp = torch.zeros(int(pstart[-1]), dtype=torch.int32, device="cuda")
     xp[dst] = x
-    outp = torch.empty(ptotal, dtype=torch.int32, device="cuda")
-    del seg, dst
+    del seg, dst, offs
+    return xp, torch.empty_like(xp), pstart
+
+
+def per_array_baselines(torch, apsep_amd, stream, xp, outp, pstart, lengths, launch_count):
+    """One solve per array: apsep_run on one persistent scratch, all arrays
+    back to back on `stream` (ms for all), and apsep_launch (allocate,
+    solve, synchronise, free: the reference's launchWSTL / launchSPT) for the
+    first `launch_count` arrays (ms per array)."""
+    nseg = lengths.shape[0]
     xb, ob = xp.data_ptr(), outp.data_ptr()
     with apsep_amd.Scratch("auto", int(lengths.max())) as s:
         sh = stream.cuda_stream
@@ -222,33 +231,64 @@ def bench_batch(args, torch, apsep_amd):
             for k in range(nseg):
                 s.run_ptr(xb + 4 * int(pstart[k]), ob + 4 * int(pstart[k]), int(lengths[k]), sh)
         per_array()
-        res["per_array_run_ms"] = timed(per_array, 1)
+        run_ms = _timed(torch, stream, per_array, 1)
+    m = min(launch_count, nseg)
 
     def per_launch():
-        for k in range(nseg):
+        for k in range(m):
             apsep_amd.launch("auto", xb + 4 * int(pstart[k]), ob + 4 * int(pstart[k]),
                              int(lengths[k]))
-    res["per_array_launch_ms"] = timed(per_launch, 1)
+    launch_ms = _timed(torch, stream, per_launch, 1) / m
+    return run_ms, launch_ms
 
-    useful = 8.0 * total
+
+def run_batch(args, torch, apsep_amd, cpu: bool):
+    """Segmented batch: 8192 segments with log-uniform lengths in [64, 2^20]
+    (about 8.7e8 keys), one class per segment.  The batched solve (the
+    autotuner's tile shape, eager and hipGraph replay) against one solve per
+    array on the same arrays (apsep_run on a persistent scratch, and
+    apsep_launch)."""
+    import numpy as np
+    x, off, lengths, cls = batch_inputs(torch, args.nseg, 64, 1 << 20, 2024)
+    total, nseg = int(off[-1]), args.nseg
+    offs = torch.from_numpy(off).cuda()
+    out = torch.empty(total, dtype=torch.int32, device="cuda")
+    res = {}
+    stream = torch.cuda.Stream()  # graphs need a capturable (non-null) stream
+    with apsep_amd.BatchPlan(total, nseg) as p, torch.cuda.stream(stream):
+        run = lambda: p.run(x, offs, out, total=total, stream=stream)
+        tuned = p.tune(x, offs, out, total=total, stream=stream)
+        res["tuned_mode"] = "graph" if tuned[1] < tuned[0] else "eager"
+        res["tile_lanes"] = p.tile_lanes
+        p.use_graph(False)
+        for _ in range(args.batch_warmup):
+            run()
+        res["eager_ms"] = _timed(torch, stream, run, args.batch_steps)
+        p.use_graph(True)
+        run()
+        res["graph_ms"] = _timed(torch, stream, run, args.batch_steps)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy() if cpu else None
+    xp, outp, pstart = _packed(torch, x, off, lengths)
+    run_ms, launch_ms = per_array_baselines(torch, apsep_amd, stream, xp, outp, pstart,
+                                            lengths, nseg if args.workload == "batch" else 1024)
     best = min(res["eager_ms"], res["graph_ms"])
+    useful = 8.0 * total
     rec = {
-        "metric": "batched useful GB/s (8192 segments, log-uniform lengths 64..2^20)",
-        "value": round(useful / (best * 1e-3) / 1e9, 1),
-        "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(best, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int32",
-        "data": "synthetic (numpy seed 2024 lengths/classes, device-generated keys)",
-        "config": {"workload": "segmented CSR batch, %d segments, %d keys" % (nseg, total),
-                   "segments": nseg, "total": total},
-        "eager_ms": round(res["eager_ms"], 3), "graph_ms": round(res["graph_ms"], 3),
+        "workload": "segmented CSR batch, %d segments, log-uniform lengths 64..2^20, "
+                    "%d keys" % (nseg, total),
+        "arrays_per_s": round(nseg / (best * 1e-3), 1),
+        "useful_GBps": round(useful / (best * 1e-3) / 1e9, 1),
+        "ms_per_batch": round(best, 4),
+        "eager_ms": round(res["eager_ms"], 4), "graph_ms": round(res["graph_ms"], 4),
         "autotuned_mode": res["tuned_mode"], "autotuned_tile_lanes": res["tile_lanes"],
-        "per_array_run_ms": round(res["per_array_run_ms"], 2),
-        "per_array_launch_ms": round(res["per_array_launch_ms"], 2),
-        "speedup_vs_per_array_run": round(res["per_array_run_ms"] / best, 2),
-        "speedup_vs_per_array_launch": round(res["per_array_launch_ms"] / best, 2),
+        "per_array_run_ms": round(run_ms, 3),
+        "per_array_launch_ms_est": round(launch_ms * nseg, 2),
+        "per_array_launch_sampled": min(nseg, nseg if args.workload == "batch" else 1024),
+        "speedup_vs_per_array_run": round(run_ms / best, 2),
+        "speedup_vs_per_array_launch": round(launch_ms * nseg / best, 2),
     }
-    if not args.no_cpu:
+    if cpu:
         import oracle
         host = x.cpu().numpy()
         threads = len(os.sched_getaffinity(0))
@@ -265,46 +305,73 @@ def bench_batch(args, torch, apsep_amd):
         rec["per_array_parity_checked"] = bool(all(
             np.array_equal(outp_h[pstart[k]:pstart[k] + lengths[k]], got[off[k]:off[k + 1]])
             for k in range(0, nseg, 97)))
-    print(json.dumps(rec), flush=True)
+    del x, out, xp, outp
+    torch.cuda.empty_cache()
+    return rec
 
 
-def bench_serve(args, torch, apsep_amd):
-    """Served stream of independent jobs: Poisson arrivals at swept rates,
-    lengths log-uniform in [2^10, 2^24], classes 60/20/20 random /
-    descending / ascending, a 64 GB device pool.  Jobs' keys are resident in
-    device memory when they arrive (a fixed set of distinct payloads, reused
-    round-robin); latency runs from the scheduled arrival (so a job refused
-    by admission waits in the client backlog) to the moment the server
-    observes its batch finished."""
+def bench_batch(args, torch, apsep_amd):
+    rec = run_batch(args, torch, apsep_amd, not args.no_cpu)
+    line = {
+        "metric": "batched useful GB/s (8192 segments, log-uniform lengths 64..2^20)",
+        "value": rec["useful_GBps"], "unit": "GB/s", "n_gpus": 1,
+        "steps": args.batch_steps, "warmup": args.batch_warmup,
+        "ms_per_step": rec["ms_per_batch"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (numpy seed 2024 lengths/classes, device-generated keys)",
+        "config": {"workload": rec["workload"], "segments": args.nseg},
+    }
+    line.update({k: v for k, v in rec.items() if k != "workload"})
+    print(json.dumps(line), flush=True)
+
+
+def serve_payloads(torch, P, seed):
+    """P distinct jobs: lengths log-uniform in [2^10, 2^24], classes 60/20/20
+    random / descending / ascending, keys generated on the device into one
+    buffer (16-byte aligned slots)."""
     import numpy as np
-    rng = np.random.default_rng(7)
-    P = args.payloads
+    rng = np.random.default_rng(seed)
     lengths = np.floor(np.exp(rng.uniform(np.log(1 << 10), np.log(1 << 24), size=P))).astype(np.int64)
     cls = rng.choice(3, size=P, p=[0.6, 0.2, 0.2])
     off = np.zeros(P + 1, dtype=np.int64)
-    np.cumsum(lengths, out=off[1:])
+    np.cumsum((lengths + 3) // 4 * 4, out=off[1:])
     total = int(off[-1])
     dev = "cuda"
-    L = torch.from_numpy(lengths).to(dev)
-    seg = torch.repeat_interleave(torch.arange(P, device=dev), L)
+    Lp = torch.from_numpy((lengths + 3) // 4 * 4).to(dev)
+    seg = torch.repeat_interleave(torch.arange(P, device=dev), Lp)
     local = torch.arange(total, device=dev, dtype=torch.int64) - torch.from_numpy(off[:-1]).to(dev)[seg]
+    L = torch.from_numpy(lengths).to(dev)
     c = torch.from_numpy(cls).to(dev)[seg]
     g = torch.Generator(device=dev)
-    g.manual_seed(7)
+    g.manual_seed(seed)
     rnd = torch.randint(0, 2**31 - 1, (total,), generator=g, dtype=torch.int32, device=dev)
     src = torch.where(c == 0, rnd, torch.where(c == 1, (L[seg] - local).to(torch.int32),
                                                local.to(torch.int32))).contiguous()
     del seg, local, c, rnd
     torch.cuda.synchronize()
+    return src, off, lengths, cls
+
+
+def run_serve(args, torch, apsep_amd, cpu: bool):
+    """Served stream of independent jobs: Poisson arrivals at swept offered
+    rates, lengths log-uniform in [2^10, 2^24], classes 60/20/20 random /
+    descending / ascending, a 64 GB device pool.  Jobs' keys are resident in
+    device memory when they arrive (a fixed set of distinct payloads, reused
+    round-robin); latency runs from the scheduled arrival (so a job refused
+    by admission waits in the client backlog) to the moment the server
+    observes its batch finished.  Baselines on the same job sequence: one
+    apsep_run per job back to back, and apsep_launch per job."""
+    import numpy as np
+    rng = np.random.default_rng(7)
+    P = args.payloads
+    src, off, lengths, cls = serve_payloads(torch, P, 7)
     base = src.data_ptr()
-    ptr = [base + 4 * int(off[k]) for k in range(P)]
-    lens = [int(x) for x in lengths]
     rates = [float(r) for r in args.rates.split(",")]
     J = args.jobs
     sweep = []
     sample_answers = {}
-    ptr_arr = np.array(ptr, dtype=np.uint64)
-    len_arr = np.array(lens, dtype=np.int64)
+    ptr_arr = np.array([base + 4 * int(off[k]) for k in range(P)], dtype=np.uint64)
+    len_arr = lengths.astype(np.int64)
     with apsep_amd.Server(64 << 30, 1 << 28, 4096) as srv:
         for rate in rates:
             arr = (time.monotonic_ns() / 1e3 + 2e5
@@ -315,6 +382,7 @@ def bench_serve(args, torch, apsep_amd):
             id0 = None         # id of this rate's first job (ids are consecutive)
             lat = np.zeros(J)
             refused = 0
+            t_last = arr[0]
             while rel < J:
                 now = time.monotonic_ns() / 1e3
                 m = int(np.searchsorted(arr, now, side="right"))
@@ -330,9 +398,9 @@ def bench_serve(args, torch, apsep_amd):
                     if d > rel:
                         _, done = srv.job_times(id0 + rel, d - rel)
                         lat[rel:d] = done - arr[rel:d]
-                        if rate == rates[-1]:
+                        if rate == rates[-1] and cpu:
                             for j in range(rel, d):
-                                if j % 997 == 0:
+                                if j % 97 == 0:
                                     sample_answers[j % P] = srv.answers(id0 + j).cpu().numpy()
                         t_last = float(done[-1])
                         srv.release_upto(id0 + d)
@@ -353,33 +421,78 @@ def bench_serve(args, torch, apsep_amd):
                           "admission_refusals": int(refused)})
             print("serve rate %.0f/s: %s" % (rate, json.dumps(sweep[-1])), file=sys.stderr,
                   flush=True)
+    # the same job sequence, one plain solve per job
+    which = np.arange(J) % P
+    stream = torch.cuda.current_stream()
+    outp = torch.empty_like(src)
+    xb, ob = src.data_ptr(), outp.data_ptr()
+    with apsep_amd.Scratch("auto", int(lengths.max())) as s:
+        sh = stream.cuda_stream
+
+        def per_job():
+            for j in range(J):
+                k = which[j]
+                s.run_ptr(xb + 4 * int(off[k]), ob + 4 * int(off[k]), int(lengths[k]), sh)
+        per_job()
+        run_ms = _timed(torch, stream, per_job, 1)
+    m = min(J, 500)
+
+    def per_launch():
+        for j in range(m):
+            k = which[j]
+            apsep_amd.launch("auto", xb + 4 * int(off[k]), ob + 4 * int(off[k]),
+                             int(lengths[k]))
+    launch_ms = _timed(torch, stream, per_launch, 1) / m
     best = max(sweep, key=lambda r: r["jobs_per_s"])
+    run_jps = J / (run_ms * 1e-3)
+    launch_jps = 1.0 / (launch_ms * 1e-3)
     rec = {
-        "metric": "served jobs/s (Poisson arrivals, lengths log-uniform 2^10..2^24, 64 GB pool)",
-        "value": best["jobs_per_s"], "unit": "jobs/s", "n_gpus": 1, "steps": J,
-        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int32",
-        "data": "synthetic (numpy seed 7 lengths/classes/arrivals, device-generated keys, "
-                "%d distinct payloads reused round-robin)" % P,
-        "config": {"workload": "served stream, %d jobs per rate, mean %.0f keys"
-                               % (J, float(lengths.mean())), "rates": rates},
+        "workload": "served stream, %d jobs per rate, %d distinct payloads, mean %.0f keys, "
+                    "64 GB pool" % (J, P, float(lengths[which].mean())),
+        "best_jobs_per_s": best["jobs_per_s"],
+        "best_useful_GBps": best["useful_GBps"],
         "sweep": sweep,
+        "per_job_run_jobs_per_s": round(run_jps, 1),
+        "per_job_launch_jobs_per_s": round(launch_jps, 1),
+        "per_job_launch_sampled": m,
+        "speedup_vs_per_job_run": round(best["jobs_per_s"] / run_jps, 2),
+        "speedup_vs_per_job_launch": round(best["jobs_per_s"] / launch_jps, 2),
     }
-    if not args.no_cpu:
+    if cpu:
         import oracle
         host = src.cpu().numpy()
+        hl = np.zeros(P + 1, dtype=np.int64)
+        np.cumsum(lengths, out=hl[1:])
+        keys = np.concatenate([host[off[k]:off[k] + lengths[k]] for k in range(P)])
         threads = len(os.sched_getaffinity(0))
         t0 = time.perf_counter()
-        exp = oracle.apsep_segmented(host, off, False, threads=threads)
+        exp = oracle.apsep_segmented(keys, hl, False, threads=threads)
         secs = time.perf_counter() - t0
         rec["cpu_baseline"] = {
             "value": round(P / secs, 1), "unit": "jobs/s", "cores": threads, "kind": "port",
             "sample": "the %d distinct payloads once, as independent jobs on a %d-thread "
                       "pool (oracle/apsep_oracle.c -O3), %.2f s" % (P, threads, secs)}
-        rec["parity_checked"] = bool(all(
-            np.array_equal(v, exp[off[k]:off[k + 1]]) for k, v in sample_answers.items()))
+        rec["parity_checked"] = bool(sample_answers) and bool(all(
+            np.array_equal(v, exp[hl[k]:hl[k + 1]]) for k, v in sample_answers.items()))
         rec["parity_jobs_checked"] = len(sample_answers)
-    print(json.dumps(rec), flush=True)
+    del src, outp
+    torch.cuda.empty_cache()
+    return rec
+
+
+def bench_serve(args, torch, apsep_amd):
+    rec = run_serve(args, torch, apsep_amd, not args.no_cpu)
+    line = {
+        "metric": "served jobs/s (Poisson arrivals, lengths log-uniform 2^10..2^24, 64 GB pool)",
+        "value": rec["best_jobs_per_s"], "unit": "jobs/s", "n_gpus": 1, "steps": args.jobs,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (numpy seed 7 lengths/classes/arrivals, device-generated keys, "
+                "%d distinct payloads reused round-robin)" % args.payloads,
+        "config": {"workload": rec["workload"], "rates": args.rates},
+    }
+    line.update({k: v for k, v in rec.items() if k != "workload"})
+    print(json.dumps(line), flush=True)
 
 
 def bench_sweep(args, torch, apsep_amd):
@@ -431,72 +544,54 @@ def bench_sweep(args, torch, apsep_amd):
         "best_random_warm_GBps": best, "rows": rows}), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default: 500 for the single-array headline, whose "
-                         "step is ~0.2 ms, so that the figure is the sustained rate and "
-                         "not the first milliseconds after an idle GPU; 20 otherwise)")
-    ap.add_argument("--warmup", type=int, default=None,
-                    help="untimed steps before the timed ones (default: 50 single, 3 otherwise)")
-    ap.add_argument("--n", type=int, default=1 << 25)
-    ap.add_argument("--algo", default="auto")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
-                    help="rocprofv3 FETCH/WRITE summary (bytes per launch)")
-    ap.add_argument("--workload", default="single", choices=["single", "batch", "serve", "sweep"],
-                    help="single: the plain N=2^25 solve (headline); batch: the "
-                         "segmented CSR batch; serve: the serving engine; sweep: "
-                         "sizes 2^14..2^27, cache-warm and flushed")
-    ap.add_argument("--nseg", type=int, default=8192)
-    ap.add_argument("--jobs", type=int, default=20000, help="serve: jobs per rate")
-    ap.add_argument("--payloads", type=int, default=1000, help="serve: distinct jobs")
-    ap.add_argument("--rates", default="10000,50000,150000,400000",
-                    help="serve: offered jobs/s")
-    args = ap.parse_args()
-    if args.steps is None:
-        args.steps = 500 if args.workload == "single" else 20
-    if args.warmup is None:
-        args.warmup = 50 if args.workload == "single" else 3
-    if args.gpus != 1:
-        print("bench.py: this build measures one GPU only; --gpus %d is not "
-              "supported (the solver does not shard across GPUs)" % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
+def copy_gbs(torch, n, stream, reps=30, rotate=1):
+    """Useful GB/s of a plain device copy of n int32 (8 n bytes moved),
+    back to back over `rotate` distinct source/destination pairs in turn."""
+    pairs = [(torch.empty(n, dtype=torch.int32, device="cuda"),
+              torch.empty(n, dtype=torch.int32, device="cuda")) for _ in range(rotate)]
+    for src, dst in pairs:
+        dst.copy_(src)
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    for r in range(reps):
+        src, dst = pairs[r % rotate]
+        dst.copy_(src)
+    b.record(stream)
+    torch.cuda.synchronize()
+    return 8.0 * n * reps / (a.elapsed_time(b) * 1e-3) / 1e9
 
-    import torch
-    import apsep_amd
 
-    if args.workload == "batch":
-        bench_batch(args, torch, apsep_amd)
-        return
-    if args.workload == "serve":
-        bench_serve(args, torch, apsep_amd)
-        return
-    if args.workload == "sweep":
-        bench_sweep(args, torch, apsep_amd)
-        return
+def _kernel_us(t):
+    return {"tile_pass": round(t.tile_ms / t.runs * 1e3, 2),
+            "level_build": round(t.levels_ms / t.runs * 1e3, 2),
+            "fix_pass": round(t.fix_ms / t.runs * 1e3, 2), "launches": int(t.runs)}
 
+
+def run_single(args, torch, apsep_amd):
     n = args.n
     stream = torch.cuda.current_stream()
     xs = make_inputs(torch, n)
     outs = {k: torch.empty(n, dtype=torch.int32, device="cuda") for k in CLASSES}
     torch.cuda.synchronize()
+    per_reps = max(args.steps, 100)
     with apsep_amd.Scratch(args.algo, n) as s:
-        per, total_ms, kt = time_solves(torch, s, xs, outs, args.steps, args.warmup, stream)
+        t0 = time.perf_counter()
+        settle(torch, s, xs, outs, stream, args.settle)
+        settle_s = time.perf_counter() - t0
+        per, total_ms, rot, warm = time_solves(torch, apsep_amd, s, xs, outs, args.steps,
+                                               args.warmup, stream, per_reps)
         algo_name = {v: k for k, v in apsep_amd.ALGOS.items()}[s.algo]
-    mean = per
     useful = 8.0 * n
-    per_class = {k: round(useful / (mean[k] * 1e-3) / 1e9, 1) for k in CLASSES}
+    per_class = {k: round(useful / (per[k] * 1e-3) / 1e9, 1) for k in CLASSES}
     step_ms = total_ms / args.steps
     value = 3 * useful / (step_ms * 1e-3) / 1e9
-    runs = sum(t.runs for t in kt.values())
-    tile_ms = sum(t.tile_ms for t in kt.values()) / runs
-    fix_ms = sum(t.fix_ms for t in kt.values()) / runs
-    levels_ms = sum(t.levels_ms for t in kt.values()) / runs
+    tile_ms = rot.tile_ms / rot.runs
     achieved = useful / (tile_ms * 1e-3) / 1e9
+    wruns = sum(t.runs for t in warm.values())
+    wtile_ms = sum(t.tile_ms for t in warm.values()) / wruns
+    wachieved = useful / (wtile_ms * 1e-3) / 1e9
 
     traffic = None
     if os.path.exists(args.traffic):
@@ -526,21 +621,22 @@ def main():
                                "3 classes per step (random/descending/ascending)",
                    "n": n, "algo": algo_name},
         "pct_of_hbm_peak": round(100.0 * value / PEAK_HBM_GBS, 2),
+        "settle_s": round(settle_s, 2),
         "per_class_gbs": per_class,
-        "per_class_us": {k: round(mean[k] * 1e3, 2) for k in CLASSES},
+        "per_class_us": {k: round(per[k] * 1e3, 2) for k in CLASSES},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel": "k_tile_pass (mean over the 3 classes)",
-                     "algorithmic_bytes_per_launch": int(useful)},
-        "kernel_us": {"tile_pass": round(tile_ms * 1e3, 2),
-                      "level_build": round(levels_ms * 1e3, 2),
-                      "fix_pass": round(fix_ms * 1e3, 2), "launches": int(runs),
-                      "tile_pass_per_class": {k: round(t.tile_ms / t.runs * 1e3, 2)
-                                              for k, t in kt.items()},
-                      "fix_pass_per_class": {k: round(t.fix_ms / t.runs * 1e3, 2)
-                                             for k, t in kt.items()}},
-        "copy_gbs": round(copy_gbs(torch, n, stream), 1),
+                     "kernel": "k_tile_pass (mean over the 3 classes in turn, one "
+                               "scratch/input/output per class)",
+                     "algorithmic_bytes_per_launch": int(useful),
+                     "warm_achieved": round(wachieved, 1),
+                     "warm_frac": round(wachieved / PEAK_HBM_GBS, 4)},
+        "kernel_us": _kernel_us(rot),
+        "kernel_us_warm": dict(_kernel_us_from(warm), per_class={
+            k: _kernel_us(t) for k, t in warm.items()}),
+        "copy_gbs": round(copy_gbs(torch, n, stream, rotate=3), 1),
+        "copy_gbs_warm": round(copy_gbs(torch, n, stream, rotate=1), 1),
     }
     if not args.no_cpu:
         cb, ok = cpu_baseline(torch, xs, outs, args.cpu_seconds)
@@ -548,6 +644,87 @@ def main():
         rec["parity_checked"] = ok
         if not ok:
             print("bench.py: GPU answers differ from the oracle", file=sys.stderr)
+    del xs, outs
+    torch.cuda.empty_cache()
+    return rec
+
+
+def _kernel_us_from(times):
+    runs = sum(t.runs for t in times.values())
+    return {"tile_pass": round(sum(t.tile_ms for t in times.values()) / runs * 1e3, 2),
+            "level_build": round(sum(t.levels_ms for t in times.values()) / runs * 1e3, 2),
+            "fix_pass": round(sum(t.fix_ms for t in times.values()) / runs * 1e3, 2),
+            "launches": int(runs)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 500 for the single-array headline, whose "
+                         "step is ~0.2 ms; 20 otherwise)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps before the timed ones (default: 50 single, 3 otherwise)")
+    ap.add_argument("--settle", type=float, default=1.0,
+                    help="seconds of untimed solves before the warmup steps, so that the "
+                         "GPU runs at its sustained clock")
+    ap.add_argument("--n", type=int, default=1 << 25)
+    ap.add_argument("--algo", default="auto")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="single: leave out the batch and serve sections")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
+                    help="rocprofv3 FETCH/WRITE summary (bytes per launch)")
+    ap.add_argument("--workload", default="single", choices=["single", "batch", "serve", "sweep"],
+                    help="single: the plain N=2^25 solve (headline) with the batch and "
+                         "serve sections; batch: the segmented CSR batch alone; serve: "
+                         "the serving engine alone; sweep: sizes 2^14..2^27, cache-warm "
+                         "and flushed")
+    ap.add_argument("--nseg", type=int, default=8192)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-warmup", type=int, default=3)
+    ap.add_argument("--jobs", type=int, default=None,
+                    help="serve: jobs per rate (default 20000 alone, 4000 in the headline)")
+    ap.add_argument("--payloads", type=int, default=None,
+                    help="serve: distinct jobs (default 1000 alone, 400 in the headline)")
+    ap.add_argument("--rates", default="10000,50000,150000,400000",
+                    help="serve: offered jobs/s")
+    args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 500 if args.workload == "single" else 20
+    if args.warmup is None:
+        args.warmup = 50 if args.workload == "single" else 3
+    if args.jobs is None:
+        args.jobs = 20000 if args.workload == "serve" else 4000
+    if args.payloads is None:
+        args.payloads = 1000 if args.workload == "serve" else 400
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; --gpus %d is not "
+              "supported (the solver does not shard across GPUs)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import apsep_amd
+
+    if args.workload == "batch":
+        bench_batch(args, torch, apsep_amd)
+        return
+    if args.workload == "serve":
+        bench_serve(args, torch, apsep_amd)
+        return
+    if args.workload == "sweep":
+        bench_sweep(args, torch, apsep_amd)
+        return
+
+    rec = run_single(args, torch, apsep_amd)
+    print("bench: single done (%.1f GB/s)" % rec["value"], file=sys.stderr, flush=True)
+    if not args.no_extras:
+        rec["batch"] = run_batch(args, torch, apsep_amd, not args.no_cpu)
+        print("bench: batch done", file=sys.stderr, flush=True)
+        rec["serve"] = run_serve(args, torch, apsep_amd, not args.no_cpu)
+        print("bench: serve done", file=sys.stderr, flush=True)
     print(json.dumps(rec), flush=True)
 
 
