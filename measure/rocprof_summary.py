@@ -1,10 +1,10 @@
 """Summaries of rocprofv3 result databases for the committed profiles.
 
-  python -m apsep_amd.rocprof_summary kernels RESULTS.db [--command CMD]
+  python measure/rocprof_summary.py kernels RESULTS.db [--command CMD]
       per-kernel dispatch count and duration (avg / min / max, us) from a
       ``--kernel-trace`` run.
 
-  python -m apsep_amd.rocprof_summary traffic FETCH.db WRITE.db --n N
+  python measure/rocprof_summary.py traffic FETCH.db WRITE.db --n N
           [--algo wstl] [--json OUT] [--command CMD]
       HBM bytes per launch of each solver kernel from two ``--pmc`` runs
       (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch).  Both counters are
@@ -12,7 +12,7 @@
       same run (the copy moves exactly 4N bytes each way), so the result is
       in true bytes.
 
-  python -m apsep_amd.rocprof_summary counters DB [DB ...] --kernel NAME
+  python measure/rocprof_summary.py counters DB [DB ...] --kernel NAME
           [--skip K] [--command CMD]
       mean per dispatch of every counter recorded in the given ``--pmc``
       databases for the kernels whose name contains NAME (the first K

@@ -1,6 +1,6 @@
 """Repeated solves of one input class, for profiling a single class.
 
-  python -m apsep_amd.solve_class KIND [--n N] [--reps R] [--algo ALGO]
+  python measure/solve_class.py KIND [--n N] [--reps R] [--algo ALGO]
 
 KIND is random (uniform keys in [0, 2^31 - 1), device generator seeded
 2024), descending (a[i] = N - i) or ascending (a[i] = i).  The keys are
@@ -11,7 +11,12 @@ belongs to that class.
 from __future__ import annotations
 
 import argparse
+import os
 import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
 
 
 def main(argv=None) -> int:
