@@ -39,6 +39,7 @@ EXPORTS = [
     "apsep_batch_plan_create", "apsep_batch_plan_destroy", "apsep_run_segmented",
     "apsep_batch_plan_use_graph", "apsep_batch_plan_tune",
     "apsep_batch_plan_set_tile_lanes", "apsep_batch_plan_tile_lanes",
+    "apsep_batch_plan_tile_elems", "apsep_batch_plan_job_tiles", "apsep_run_jobs",
     "apsep_server_create", "apsep_server_destroy", "apsep_server_submit",
     "apsep_server_poll", "apsep_server_job_info", "apsep_server_release",
     "apsep_server_drain", "apsep_server_submit_many", "apsep_server_done_upto",
@@ -114,6 +115,9 @@ def lib() -> ctypes.CDLL:
         L.apsep_batch_plan_use_graph.argtypes = [vp, ci]
         L.apsep_batch_plan_set_tile_lanes.argtypes = [vp, i32]
         L.apsep_batch_plan_tile_lanes.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+        L.apsep_batch_plan_tile_elems.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+        L.apsep_batch_plan_job_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.apsep_run_jobs.argtypes = [vp, vp, i32, i64, vp]
         L.apsep_batch_plan_tune.argtypes = [vp, vp, vp, vp, i32, i64, vp, i32,
                                             ctypes.POINTER(ctypes.c_float),
                                             ctypes.POINTER(ctypes.c_float)]
@@ -344,6 +348,57 @@ class BatchPlan:
                                            ctypes.byref(g)), "apsep_batch_plan_tune")
         return e.value, g.value
 
+    @property
+    def tile_elems(self) -> int:
+        v = ctypes.c_int32(0)
+        _check(lib().apsep_batch_plan_tile_elems(self._h, ctypes.byref(v)),
+               "apsep_batch_plan_tile_elems")
+        return v.value
+
+    @property
+    def job_tiles(self) -> int:
+        v = ctypes.c_int64(0)
+        _check(lib().apsep_batch_plan_job_tiles(self._h, ctypes.byref(v)),
+               "apsep_batch_plan_job_tiles")
+        return v.value
+
+    def job_table(self, xs, outs):
+        """The apsep_job table (int64 tensor of 4 words per job, on the
+        device) and the total tile count for solving xs[k] into outs[k]
+        in place with this plan's current tile shape."""
+        import torch
+        if len(xs) != len(outs):
+            raise ValueError("apsep: one answer tensor per key tensor")
+        te = self.tile_elems
+        rows, first = [], 0
+        for x, o in zip(xs, outs):
+            n = x.numel()
+            if x.dtype != torch.int32 or o.dtype != torch.int32:
+                raise TypeError("apsep: int32 tensors required")
+            if o.numel() < n or not (x.is_contiguous() and o.is_contiguous()):
+                raise ValueError("apsep: contiguous answers of at least the keys' size required")
+            if not (x.is_cuda and o.is_cuda) or {_device_index(x), _device_index(o)} != {self.device}:
+                raise ValueError("apsep: device tensors on the plan's device required")
+            if n and (x.data_ptr() % 16 or o.data_ptr() % 16):
+                raise ValueError("apsep: job keys and answers must be 16-byte aligned")
+            rows.append([x.data_ptr(), o.data_ptr(), n, first])
+            first += (n + te - 1) // te
+        table = torch.tensor(rows, dtype=torch.int64).reshape(-1, 4).to(f"cuda:{self.device}")
+        return table, first
+
+    def run_jobs(self, xs, outs, stream=None, table=None, tiles: int | None = None):
+        """Solve each xs[k] (int32 device tensor) into outs[k] in place, all in
+        one launch sequence (apsep_run_jobs).  Returns the job table used,
+        which must stay alive until the solve has run."""
+        import torch
+        if table is None:
+            table, tiles = self.job_table(xs, outs)
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        _check(lib().apsep_run_jobs(self._h, table.data_ptr(), table.shape[0], tiles,
+                                    stream.cuda_stream), "apsep_run_jobs")
+        return table
+
     def run_ptr(self, d_in: int, d_offsets: int, d_out: int, nseg: int, total: int,
                 stream: int = 0):
         _check(lib().apsep_run_segmented(self._h, d_in, d_offsets, d_out, nseg, total,
@@ -397,8 +452,9 @@ def _hip():
 
 
 class Server:
-    """Serving engine (apsep_server_*): a ``pool_bytes`` device pool, budget
-    admission and continuous batching into segmented solves."""
+    """Serving engine (apsep_server_*): a ``pool_bytes`` device pool for the
+    answers, budget admission and continuous batching into in-place job
+    batches on two streams (keys are read where the caller holds them)."""
 
     def __init__(self, pool_bytes: int, max_batch: int, max_batch_jobs: int = 4096,
                  incl: bool = False):

@@ -271,9 +271,11 @@ apsep_status apsep_batch_plan_create(int64_t total_cap, int32_t nseg_cap, int in
     }
     const int64_t tiles = (total_cap + tile - 1) / tile;
     if (tiles > 0 && (hipMalloc((void**)&p->tmap, (size_t)tiles * sizeof(int)) != hipSuccess ||
-                      hipMalloc((void**)&p->tbound, (size_t)tiles * sizeof(int2)) != hipSuccess)) {
+                      hipMalloc((void**)&p->tbound, (size_t)tiles * sizeof(int2)) != hipSuccess ||
+                      hipMalloc((void**)&p->pages, (size_t)tiles * sizeof(Page)) != hipSuccess)) {
         APSEP_CLEAR_HIP_ERROR();
         if (p->tmap) (void)hipFree(p->tmap);
+        if (p->tbound) (void)hipFree(p->tbound);
         apsep_scratch_destroy(p->s);
         delete p;
         return APSEP_ERR_ALLOC;
@@ -294,6 +296,7 @@ apsep_status apsep_batch_plan_destroy(apsep_batch_plan* p) {
     drop_graph(p);
     if (p->tmap) (void)hipFree(p->tmap);
     if (p->tbound) (void)hipFree(p->tbound);
+    if (p->pages) (void)hipFree(p->pages);
     apsep_scratch_destroy(p->s);
     delete p;
     return APSEP_OK;
@@ -303,6 +306,59 @@ static apsep_status plan_run(apsep_batch_plan* p, const int32_t* d_in, const int
                              int32_t* d_out, int nseg, int total, hipStream_t st) {
     return p->tile_lanes == 512 ? seg512::seg_run(p, d_in, offs, d_out, nseg, total, st)
                                 : seg256::seg_run(p, d_in, offs, d_out, nseg, total, st);
+}
+
+static apsep_status plan_run_jobs(apsep_batch_plan* p, const apsep_job* jobs, int count,
+                                  int tiles, hipStream_t st) {
+    return p->tile_lanes == 512 ? seg512::jobs_run(p, jobs, count, tiles, st)
+                                : seg256::jobs_run(p, jobs, count, tiles, st);
+}
+
+// Replay the plan's graph when it was recorded for the same call (kind 0:
+// apsep_run_segmented, 1: apsep_run_jobs; same pointers, counts and
+// stream), else record it first.
+static apsep_status graph_run(apsep_batch_plan* p, int kind, const void* a, const void* b,
+                              void* c, int32_t count, int64_t total, void* stream) {
+    hipStream_t hs = (hipStream_t)stream;
+    const bool same = p->exec && p->g_jobs == kind && p->g_in == a && p->g_offs == b &&
+                      p->g_out == c && p->g_nseg == count && p->g_total == total &&
+                      p->g_stream == stream;
+    if (!same) {
+        drop_graph(p);
+        if (hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            APSEP_CLEAR_HIP_ERROR();
+            return APSEP_ERR_LAUNCH;
+        }
+        const apsep_status st =
+            kind ? plan_run_jobs(p, (const apsep_job*)a, count, (int)total, hs)
+                 : plan_run(p, (const int32_t*)a, (const int64_t*)b, (int32_t*)c, count,
+                            (int)total, hs);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(hs, &g);
+        if (st != APSEP_OK || ec != hipSuccess || !g) {
+            APSEP_CLEAR_HIP_ERROR();
+            if (g) (void)hipGraphDestroy(g);
+            return APSEP_ERR_LAUNCH;
+        }
+        p->graph = g;
+        if (hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            APSEP_CLEAR_HIP_ERROR();
+            drop_graph(p);
+            return APSEP_ERR_LAUNCH;
+        }
+        p->g_jobs = kind;
+        p->g_in = a;
+        p->g_offs = b;
+        p->g_out = c;
+        p->g_nseg = count;
+        p->g_total = total;
+        p->g_stream = stream;
+    }
+    if (hipGraphLaunch(p->exec, hs) != hipSuccess) {
+        APSEP_CLEAR_HIP_ERROR();
+        return APSEP_ERR_LAUNCH;
+    }
+    return APSEP_OK;
 }
 
 apsep_status apsep_batch_plan_set_tile_lanes(apsep_batch_plan* p, int32_t lanes) {
@@ -315,6 +371,29 @@ apsep_status apsep_batch_plan_set_tile_lanes(apsep_batch_plan* p, int32_t lanes)
 apsep_status apsep_batch_plan_tile_lanes(const apsep_batch_plan* p, int32_t* lanes) {
     if (!p || !lanes) return APSEP_ERR_INVALID_ARG;
     *lanes = p->tile_lanes;
+    return APSEP_OK;
+}
+
+static int plan_tile_elems(const apsep_batch_plan* p) {
+    return p->tile_lanes == 512 ? seg512::seg_tile_elems() : seg256::seg_tile_elems();
+}
+
+apsep_status apsep_batch_plan_tile_elems(const apsep_batch_plan* p, int32_t* elems) {
+    if (!p || !elems) return APSEP_ERR_INVALID_ARG;
+    *elems = plan_tile_elems(p);
+    return APSEP_OK;
+}
+
+// Tiles a job batch of this plan may span with its current tile shape: the
+// scratch covers total_cap keys in tiles of either shape.
+static int64_t plan_job_tiles(const apsep_batch_plan* p) {
+    const int64_t te = plan_tile_elems(p);
+    return (p->total_cap + te - 1) / te;
+}
+
+apsep_status apsep_batch_plan_job_tiles(const apsep_batch_plan* p, int64_t* tiles) {
+    if (!p || !tiles) return APSEP_ERR_INVALID_ARG;
+    *tiles = plan_job_tiles(p);
     return APSEP_OK;
 }
 
@@ -344,41 +423,24 @@ apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
     hipStream_t hs = (hipStream_t)stream;
     if (!p->use_graph || !stream)
         return plan_run(p, d_in, d_offsets, d_out, nseg, (int)total, hs);
-    const bool same = p->exec && p->g_in == d_in && p->g_offs == d_offsets &&
-                      p->g_out == d_out && p->g_nseg == nseg && p->g_total == total &&
-                      p->g_stream == stream;
-    if (!same) {
-        drop_graph(p);
-        if (hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-            APSEP_CLEAR_HIP_ERROR();
-            return APSEP_ERR_LAUNCH;
-        }
-        st = plan_run(p, d_in, d_offsets, d_out, nseg, (int)total, hs);
-        hipGraph_t g = nullptr;
-        const hipError_t ec = hipStreamEndCapture(hs, &g);
-        if (st != APSEP_OK || ec != hipSuccess || !g) {
-            APSEP_CLEAR_HIP_ERROR();
-            if (g) (void)hipGraphDestroy(g);
-            return APSEP_ERR_LAUNCH;
-        }
-        p->graph = g;
-        if (hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0) != hipSuccess) {
-            APSEP_CLEAR_HIP_ERROR();
-            drop_graph(p);
-            return APSEP_ERR_LAUNCH;
-        }
-        p->g_in = d_in;
-        p->g_offs = d_offsets;
-        p->g_out = d_out;
-        p->g_nseg = nseg;
-        p->g_total = total;
-        p->g_stream = stream;
-    }
-    if (hipGraphLaunch(p->exec, hs) != hipSuccess) {
+    return graph_run(p, 0, d_in, d_offsets, d_out, nseg, total, stream);
+}
+
+apsep_status apsep_run_jobs(apsep_batch_plan* p, const apsep_job* d_jobs, int32_t count,
+                            int64_t tiles, void* stream) {
+    if (!p || count < 0 || tiles < 0) return APSEP_ERR_INVALID_ARG;
+    if (count == 0 || tiles == 0) return APSEP_OK;
+    if (!d_jobs || ((uintptr_t)d_jobs & 7)) return !d_jobs ? APSEP_ERR_INVALID_ARG : APSEP_ERR_ALIGN;
+    if (count > p->nseg_cap || tiles > plan_job_tiles(p)) return APSEP_ERR_CAPACITY;
+    if (tiles * plan_tile_elems(p) > INT32_MAX) return APSEP_ERR_TOO_LARGE;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != p->s->device) {
         APSEP_CLEAR_HIP_ERROR();
-        return APSEP_ERR_LAUNCH;
+        return APSEP_ERR_DEVICE;
     }
-    return APSEP_OK;
+    hipStream_t hs = (hipStream_t)stream;
+    if (!p->use_graph || !stream) return plan_run_jobs(p, d_jobs, count, (int)tiles, hs);
+    return graph_run(p, 1, d_jobs, nullptr, nullptr, count, tiles, stream);
 }
 
 apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,

@@ -1243,6 +1243,39 @@ __device__ __forceinline__ int range_min_lane(const Levels& L, int a, int b) {
     return m;
 }
 
+// Where the keys and answers of a tile live, addressed by the tile's global
+// (virtual) element indices.  FlatMap: one array for every tile.  PagedMap:
+// batches of independent arrays read in place: tile t's keys start at
+// pages[t].src and its answers at pages[t].dst (each array starts on a tile
+// boundary of the virtual index space), so src(t)[t * TILE + i] is key i of
+// the tile.  The returned base is formed by integer arithmetic and only
+// dereferenced inside tile t (or, for src, the tile before it in the same
+// array).
+struct Page {
+    const int* src;
+    int* dst;
+};
+
+struct FlatMap {
+    const int* in;
+    int* out;
+    __device__ __forceinline__ const int* src(int) const { return in; }
+    __device__ __forceinline__ int* dst(int) const { return out; }
+};
+
+struct PagedMap {
+    const Page* pages;
+    __device__ __forceinline__ static uintptr_t virt(const void* p, int t) {
+        return (uintptr_t)p - (uintptr_t)t * (TILE * sizeof(int));
+    }
+    __device__ __forceinline__ const int* src(int t) const {
+        return reinterpret_cast<const int*>(virt(pages[t].src, t));
+    }
+    __device__ __forceinline__ int* dst(int t) const {
+        return reinterpret_cast<int*>(virt(pages[t].dst, t));
+    }
+};
+
 // Min of in[s0 .. t * TILE) for s0 < t * TILE (whole wave, uniform result):
 // the rest of s0's chunk, the rest of its tile's chunk mins, then whole
 // tiles through the hierarchy.  Tile s0 / TILE precedes t, so it is complete.
@@ -1267,11 +1300,12 @@ __device__ __forceinline__ int seg_prefix_min(const int* __restrict__ in, const 
 
 // CO: the fix-up runs in the persistent launch that wrote the summaries
 // (tile mins, keys, spines, chunk mins are read coherently; the tile's own
-// open list and outputs were written by this workgroup).
-template <bool INCL, bool SEG = false, bool CO = false>
-__device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __restrict__ out,
-                                         int n, int t, const Levels& L, const TileOut& to,
-                                         int* lds, const SegIn sg = SegIn{nullptr, nullptr, 0, nullptr}) {
+// open list and outputs were written by this workgroup).  io: where each
+// tile's keys and answers live (FlatMap or PagedMap).
+template <bool INCL, bool SEG = false, bool CO = false, class Map = FlatMap>
+__device__ __forceinline__ void fix_tile(const Map& io, int n, int t, const Levels& L,
+                                         const TileOut& to, int* lds,
+                                         const SegIn sg = SegIn{nullptr, nullptr, 0, nullptr}) {
     static_assert(!(SEG && CO), "segmented solves run as separate launches");
     if (t == 0) return;
     const int lane = lane_id();
@@ -1286,7 +1320,9 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
     const int sc1 = ld_i<CO>(&to.scount[t - 1]);
     const int sc2 = (t >= 2) ? ld_i<CO>(&to.scount[t - 2]) : 0;
     const int cnt = ld_i<CO>(&to.qcount[t]);
-    const int v1 = in[tile_start];
+    // the tile's first key; a tile of a paged batch that no job covers has
+    // no keys (and no page: nothing of it is read)
+    const int v1 = (!SEG || tb.y > tile_start) ? io.src(t)[tile_start] : INF;
     // the list slots are read with the rest of the first batch, before the
     // count is known; slots past the count are ignored below
     int2 e = to.qlist[(size_t)t * QS + (lane & (QS - 1))];
@@ -1335,7 +1371,7 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
             for (int i = lane; i < b1; i += WAVE) p = min(p, key_value(ld_u64<CO>(&key_at(L, 1, i)), L.epoch));
         }
     }
-    const int P = SEG ? seg_prefix_min(in, to, L, s0, t) : wave_min(p);
+    const int P = SEG ? seg_prefix_min(io.src(s0 / TILE), to, L, s0, t) : wave_min(p);
     if (!qual<INCL>(P, v1)) return;  // every open element answers -1
     int* win = lds;
     int* gm = lds + WAVE;  // min of window tiles 4g .. 4g+3
@@ -1408,11 +1444,12 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
         if (active) {
             int r;
             if (ts < 0) r = -1;  // unreachable: P qualifies, so some tile does
-            else if (ts == t - 1) r = answer_in_tile<INCL, CO>(in, to, ts, v, sp, sc1);
-            else if (ts == t - 2) r = answer_in_tile<INCL, CO>(in, to, ts, v, sp + RS, sc2);
-            else r = answer_in_tile<INCL, CO>(in, to, ts, v, nullptr, ld_i<CO>(&to.scount[ts]));
+            else if (ts == t - 1) r = answer_in_tile<INCL, CO>(io.src(ts), to, ts, v, sp, sc1);
+            else if (ts == t - 2) r = answer_in_tile<INCL, CO>(io.src(ts), to, ts, v, sp + RS, sc2);
+            else r = answer_in_tile<INCL, CO>(io.src(ts), to, ts, v, nullptr,
+                                              ld_i<CO>(&to.scount[ts]));
             if (SEG) r = r >= s0 ? r - s0 : -1;
-            out[tile_start + e.x] = r;
+            io.dst(t)[tile_start + e.x] = r;
         }
         return;
     }
@@ -1427,7 +1464,7 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
         const int p0 = tile_start + o;
         const bool valid = lane < span - o;
         const int idx = valid ? p0 + lane : p0;
-        const int xv = valid ? in[idx] : INF;
+        const int xv = valid ? io.src(t)[idx] : INF;
         const int inc = wave_incl_min(xv);
         int ex = wave_shfl_up(inc, 1);
         const bool ex_real = lane > 0 || have;
@@ -1435,10 +1472,10 @@ __device__ __forceinline__ void fix_tile(const int* __restrict__ in, int* __rest
         const bool open = valid && !(ex_real && qual<INCL>(ex, xv));
         if (open && qual<INCL>(P, xv)) {
             const int ts = find_tile<INCL, CO>(L, t, xv);
-            int r = (ts < 0) ? -1 : answer_in_tile<INCL, CO>(in, to, ts, xv, nullptr,
+            int r = (ts < 0) ? -1 : answer_in_tile<INCL, CO>(io.src(ts), to, ts, xv, nullptr,
                                                             ld_i<CO>(&to.scount[ts]));
             if (SEG) r = r >= s0 ? r - s0 : -1;
-            out[idx] = r;
+            io.dst(t)[idx] = r;
         }
         rm = min(rm, __shfl(inc, WAVE - 1, WAVE));
         have = true;

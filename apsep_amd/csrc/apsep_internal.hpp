@@ -44,6 +44,7 @@ struct apsep_batch_plan {
     int32_t nseg_cap;
     int* tmap;             // [tiles at capacity]
     int2* tbound;          // [tiles at capacity]: that segment's start and end
+    apsep::Page* pages;    // [tiles at capacity]: in-place job batches
     int tile_lanes;        // tile shape the batches run with: 256 or 512 lanes
     int use_graph;
     hipGraph_t graph;
@@ -55,6 +56,7 @@ struct apsep_batch_plan {
     int32_t g_nseg;
     int64_t g_total;
     void* g_stream;
+    int g_jobs;            // the graph holds a job batch (apsep_run_jobs)
 };
 
 namespace apsep {
@@ -89,11 +91,15 @@ apsep_status noalloc_run(const int32_t* in, int32_t* out, int n, int incl, hipSt
 namespace seg256 {
 apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs,
                      int32_t* out, int nseg, int n, hipStream_t st);
+apsep_status jobs_run(apsep_batch_plan* p, const void* jobs, int njobs, int tiles,
+                      hipStream_t st);
 int seg_tile_elems();
 }  // namespace seg256
 namespace seg512 {
 apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs,
                      int32_t* out, int nseg, int n, hipStream_t st);
+apsep_status jobs_run(apsep_batch_plan* p, const void* jobs, int njobs, int tiles,
+                      hipStream_t st);
 int seg_tile_elems();
 }  // namespace seg512
 

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(SEG_FIX_WG) __attribute__((amdgpu_waves_per_eu(8)))
     __shared__ int lds[SEG_FIX_WG / WAVE][FIX_LDS_INTS];
     const int t = (int)(blockIdx.x * (SEG_FIX_WG / WAVE) + (threadIdx.x >> 6));
     if (t >= L.nt) return;
-    fix_tile<INCL, true>(in, out, n, t, L, to, lds[threadIdx.x >> 6], sg);
+    fix_tile<INCL, true>(FlatMap{in, out}, n, t, L, to, lds[threadIdx.x >> 6], sg);
 }
 
 template <bool INCL>
@@ -140,6 +140,129 @@ static apsep_status seg_launch(apsep_batch_plan* p, const int* in, const long lo
                            st, in, out, n, L, to, sg);
     }
     return APSEP_LAUNCH_FAILED() ? APSEP_ERR_LAUNCH : APSEP_OK;
+}
+
+// ---- Batches of arrays read in place (apsep_run_jobs) ----------------------
+// Job k's keys and answers stay in the caller's memory.  In the solver's
+// virtual index space job k occupies whole tiles from first_tile, so no tile
+// holds two jobs: a page per tile gives its keys' and answers' addresses,
+// and the tile's segment bound (tbound) is its job's first and last virtual
+// index.  Everything else is the segmented solve (answers rebased to the
+// job's start, prefix mins and searches bounded by it).
+struct JobDesc {          // apsep_job
+    const int* keys;
+    int* answers;
+    long long n;
+    long long first_tile;
+};
+
+// Page and bound of every tile: tile t belongs to the last job whose
+// first_tile <= t (binary search, so each tile is written exactly once); a
+// tile no job covers becomes an empty tile (bound [t*TILE, t*TILE)).  Also
+// empties level 1.
+__global__ __launch_bounds__(256) void k_jobs_setup(const JobDesc* __restrict__ jobs, int njobs,
+                                                    Page* __restrict__ pages,
+                                                    int2* __restrict__ tbound, int nt,
+                                                    u64* __restrict__ key1, int nkey1) {
+    const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int stride = (int)(gridDim.x * blockDim.x);
+    for (int i = gid; i < nkey1; i += stride) key1[key_slot(1, i)] = ~0ull;
+    for (int t = gid; t < nt; t += stride) {
+        int lo = 0, hi = njobs - 1;  // last job with first_tile <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].first_tile <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const JobDesc j = jobs[lo];
+        const long long local = (long long)t - j.first_tile;  // tile within the job
+        const long long ntl = j.n > 0 ? (j.n + TILE - 1) / TILE : 0;
+        const int ts = t * TILE;
+        if (local >= 0 && local < ntl) {
+            pages[t] = Page{j.keys + local * TILE, j.answers + local * TILE};
+            const long long s = j.first_tile * TILE;
+            tbound[t] = make_int2((int)s, (int)min(s + j.n, (long long)INT_MAX));
+        } else {
+            pages[t] = Page{nullptr, nullptr};
+            tbound[t] = make_int2(ts, ts);
+        }
+    }
+}
+
+template <bool INCL>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 256 ? 5 : 6))) void k_tile_pass_jobs(const Page* __restrict__ pages,
+                                                       const int2* __restrict__ tbound,
+                                                       TileOut to, Levels L) {
+    __shared__ TileSmem sm;
+    const int t = (int)blockIdx.x;
+    const Page pg = pages[t];
+    const int2 b = tbound[t];
+    const int tile_start = t * TILE;
+    // the tile's valid keys end at its job's end (tile_start when empty)
+    const int n = max(min(b.y, tile_start + TILE), tile_start);
+    const int* in = reinterpret_cast<const int*>(PagedMap::virt(pg.src, t));
+    int* out = reinterpret_cast<int*>(PagedMap::virt(pg.dst, t));
+    int v[K];
+    load_lane(in, n, t, v);
+    // the key before the tile, when the tile is not its job's first
+    const int pl = (threadIdx.x == 0 && b.x < tile_start) ? in[tile_start - 1] : 0;
+    solve_tile<INCL, true, false>(out, n, t, v, pl, sm, to, L, SegLane{b.x, 0u});
+}
+
+template <bool INCL>
+__global__ __launch_bounds__(SEG_FIX_WG) __attribute__((amdgpu_waves_per_eu(8))) void k_fix_pass_jobs(const Page* __restrict__ pages, int n,
+                                                              Levels L, TileOut to,
+                                                              SegIn sg) {
+    __shared__ int lds[SEG_FIX_WG / WAVE][FIX_LDS_INTS];
+    const int t = (int)(blockIdx.x * (SEG_FIX_WG / WAVE) + (threadIdx.x >> 6));
+    if (t >= L.nt) return;
+    fix_tile<INCL, true>(PagedMap{pages}, n, t, L, to, lds[threadIdx.x >> 6], sg);
+}
+
+template <bool INCL>
+static apsep_status jobs_launch(apsep_batch_plan* p, const JobDesc* jobs, int njobs, int nt,
+                                hipStream_t st) {
+    apsep_scratch* s = p->s;
+    Levels L;
+    int sz[MAX_LEVELS];
+    L.nt = nt;
+    L.nlev = level_sizes(nt, sz);
+    L.mn0 = s->mn0;
+    L.upper = (L.nlev > 2 && sz[1] > UPPER_MIN) ? 1 : 0;
+    for (int l = 0; l < MAX_LEVELS; l++) {
+        L.n[l] = l < L.nlev ? sz[l] : 0;
+        L.key[l] = (l >= 1 && l < L.nlev) ? s->key[l] : nullptr;
+    }
+    L.epoch = SEG_EPOCH;
+    TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
+    SegIn sg{nullptr, nullptr, njobs, p->tbound};
+    const int nkey1 = L.nlev > 1 ? sz[1] : 0;
+    const int work = nt > nkey1 ? nt : nkey1;
+    int blocks = (work + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_jobs_setup, dim3(blocks), dim3(256), 0, st, jobs, njobs, p->pages,
+                       p->tbound, nt, nkey1 ? s->key[1] : nullptr, nkey1);
+    hipLaunchKernelGGL(k_tile_pass_jobs<INCL>, dim3(nt), dim3(NT), 0, st, p->pages, p->tbound,
+                       to, L);
+    if (nt > 1) {
+        if (L.nlev > 1) hipLaunchKernelGGL(k_level1_seg, dim3((L.n[1] + 3) / 4), dim3(256), 0, st, L);
+        if (L.upper)
+            for (int l = 2; l < L.nlev; l++)
+                hipLaunchKernelGGL(k_level_seg, dim3((L.n[l] + 3) / 4), dim3(256), 0, st, L, l);
+        constexpr int tpw = SEG_FIX_WG / WAVE;
+        hipLaunchKernelGGL(k_fix_pass_jobs<INCL>, dim3((nt + tpw - 1) / tpw), dim3(SEG_FIX_WG), 0,
+                           st, p->pages, nt * TILE, L, to, sg);
+    }
+    return APSEP_LAUNCH_FAILED() ? APSEP_ERR_LAUNCH : APSEP_OK;
+}
+
+apsep_status jobs_run(apsep_batch_plan* p, const void* jobs, int njobs, int tiles,
+                      hipStream_t st) {
+    static_assert(sizeof(JobDesc) == 32, "apsep_job layout");
+    const JobDesc* j = reinterpret_cast<const JobDesc*>(jobs);
+    return p->s->incl ? jobs_launch<true>(p, j, njobs, tiles, st)
+                      : jobs_launch<false>(p, j, njobs, tiles, st);
 }
 
 apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs,

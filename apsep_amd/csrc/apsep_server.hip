@@ -1,25 +1,24 @@
 // apsep_server.hip -- host-side serving engine: a fixed HBM pool, budget
 // admission, and continuous batching of independent solve jobs into
-// segmented launches.
+// in-place job batches on several streams.
 //
 // The reference serves nothing: each launchWSTL / launchSPT call allocates
 // its scratch, solves one array and synchronises the device
 // (src/apsep.cuh:2598-2605, 3069-3076).  Here:
-//   pool       one hipMalloc split into a key ring and an answer ring with
-//              the same indexing; a job owns a contiguous slot in both,
-//              allocated in submission order (16-byte aligned)
+//   pool       one hipMalloc holding the answers: a job owns a contiguous,
+//              16-byte aligned slot of it, allocated in submission order
+//              (a ring); its keys stay where the client holds them and are
+//              read in place (no copy into the pool)
 //   admission  a job is accepted only if its slot fits in the free part of
 //              the ring (APSEP_ERR_BUDGET otherwise: nothing is queued, the
 //              caller retries after releasing finished jobs)
 //   batching   poll() retires finished batches (event queries, never a
-//              blocking wait) and, while fewer than INFLIGHT batches are
-//              queued on the stream, packs every queued job whose slot
-//              continues the previous one into one segmented solve: the
-//              batch's offsets and job descriptors are uploaded from pinned
-//              memory in one copy, one gather kernel copies every job's keys
-//              into its slot, and the batch is solved in place
-// Padding between slots (< 4 keys) is absorbed by the segment before it:
-// answers depend only on earlier keys, so a job's answers do not see it.
+//              blocking wait) and, while a stream has a free batch slot,
+//              packs the queued jobs into one in-place job batch on it:
+//              the batch's job table is uploaded from pinned memory in one
+//              copy and solved by apsep_run_jobs with that stream's plan
+//   streams    NSTREAMS streams, each with its own plan (scratch) and up to
+//              INFLIGHT batches, so consecutive batches overlap on the GPU
 // One thread drives a server; servers are independent.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,51 +32,29 @@
 
 namespace {
 
-constexpr int INFLIGHT = 2;
+constexpr int NSTREAMS = 2;
+constexpr int INFLIGHT = 2;        // batches queued per stream
+constexpr int SLOTS = NSTREAMS * INFLIGHT;
+constexpr int64_t MAX_TILE = 8192; // the larger tile shape, in keys
 
 enum JobState { JOB_QUEUED = 0, JOB_RUNNING = 1, JOB_DONE = 2 };
 
 struct Job {
     const int32_t* src;
     int64_t n;
-    int64_t slot;   // element offset in both rings
+    int64_t slot;   // element offset in the answer ring
     int64_t span;   // slot length (n rounded up to 4)
     int state;
     bool released;
     double t_submit, t_done;  // microseconds, steady clock
 };
 
-// Copy every job of a batch into its slot: one workgroup per GATHER keys of
-// the batch, the jobs overlapping it found by binary search over the slots.
-constexpr int GATHER = 4096;
-
-__global__ __launch_bounds__(256) void k_gather(const long long* __restrict__ off,
-                                                const unsigned long long* __restrict__ src,
-                                                const long long* __restrict__ len, int m,
-                                                int* __restrict__ dst, long long total) {
-    const long long p0 = (long long)blockIdx.x * GATHER;
-    if (p0 >= total) return;
-    const long long p1 = p0 + GATHER < total ? p0 + GATHER : total;
-    int lo = 0, hi = m - 1;  // last job starting at or before p0
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= p0) lo = mid;
-        else hi = mid - 1;
-    }
-    for (int k = lo; k < m && off[k] < p1; k++) {
-        const long long a = off[k] > p0 ? off[k] : p0;
-        const long long e = off[k] + len[k];
-        const long long b = e < p1 ? e : p1;
-        const int* s = reinterpret_cast<const int*>(src[k]) - off[k];
-        for (long long p = a + threadIdx.x; p < b; p += blockDim.x) dst[p] = s[p];
-    }
-}
-
-struct Inflight {
+struct Batch {
     bool busy;
+    int stream;       // index into the server's streams
     hipEvent_t done;
-    int64_t* h_off;   // pinned: offsets [m + 1], sources [m], lengths [m]
-    int64_t* d_off;
+    apsep_job* h_jobs;  // pinned
+    apsep_job* d_jobs;
     int64_t first;    // job id of the batch's first job
     int32_t count;
 };
@@ -92,20 +69,20 @@ double now_us() {
 struct apsep_server {
     int device;
     int incl;
-    hipStream_t stream;
-    int32_t* ring_in;
+    hipStream_t stream[NSTREAMS];
+    apsep_batch_plan* plan[NSTREAMS];
+    int inflight[NSTREAMS];   // batches queued per stream
+    int next_stream;          // round-robin start
     int32_t* ring_out;
     int64_t cap;          // ring elements
-    int64_t max_batch;    // elements per batch
+    int64_t max_batch;    // keys per batch
     int32_t max_segs;     // jobs per batch
-    apsep_batch_plan* plan;
     std::deque<Job> jobs; // unreleased jobs, in submission (= ring) order
     int64_t front_id;     // id of jobs.front()
     int64_t next_id;
     int64_t next_launch;  // id of the first job not yet in a batch
-    Inflight inf[INFLIGHT];
-    int inf_head;         // oldest in-flight batch
-    int inf_count;
+    Batch batch[SLOTS];
+    int busy;             // batches in flight
 };
 
 static Job* job_of(apsep_server* s, int64_t id) {
@@ -128,98 +105,118 @@ static int64_t ring_alloc(apsep_server* s, int64_t span) {
 }
 
 static void destroy_server(apsep_server* s) {
-    for (int i = 0; i < INFLIGHT; i++) {
-        if (s->inf[i].done) (void)hipEventDestroy(s->inf[i].done);
-        if (s->inf[i].h_off) (void)hipHostFree(s->inf[i].h_off);
-        if (s->inf[i].d_off) (void)hipFree(s->inf[i].d_off);
+    for (int i = 0; i < SLOTS; i++) {
+        if (s->batch[i].done) (void)hipEventDestroy(s->batch[i].done);
+        if (s->batch[i].h_jobs) (void)hipHostFree(s->batch[i].h_jobs);
+        if (s->batch[i].d_jobs) (void)hipFree(s->batch[i].d_jobs);
     }
-    if (s->plan) apsep_batch_plan_destroy(s->plan);
-    if (s->ring_in) (void)hipFree(s->ring_in);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    for (int k = 0; k < NSTREAMS; k++) {
+        if (s->plan[k]) apsep_batch_plan_destroy(s->plan[k]);
+        if (s->stream[k]) (void)hipStreamDestroy(s->stream[k]);
+    }
+    if (s->ring_out) (void)hipFree(s->ring_out);
     delete s;
 }
 
-// retire finished batches in order (non-blocking unless `wait`)
+static void finish(apsep_server* s, Batch& f) {
+    const double t = now_us();
+    for (int64_t id = f.first; id < f.first + f.count; id++) {
+        Job* j = job_of(s, id);
+        if (j && j->state == JOB_RUNNING) {
+            j->state = JOB_DONE;
+            j->t_done = t;
+        }
+    }
+    f.busy = false;
+    s->inflight[f.stream]--;
+    s->busy--;
+}
+
+// Retire finished batches, in any order (non-blocking unless `wait`: then
+// block for the oldest batch first).
 static apsep_status retire(apsep_server* s, bool wait) {
-    while (s->inf_count > 0) {
-        Inflight& f = s->inf[s->inf_head];
-        hipError_t e = wait ? hipEventSynchronize(f.done) : hipEventQuery(f.done);
-        if (e == hipErrorNotReady) {
+    if (wait && s->busy > 0) {
+        int oldest = -1;
+        for (int i = 0; i < SLOTS; i++)
+            if (s->batch[i].busy && (oldest < 0 || s->batch[i].first < s->batch[oldest].first))
+                oldest = i;
+        if (hipEventSynchronize(s->batch[oldest].done) != hipSuccess) {
             APSEP_CLEAR_HIP_ERROR();
-            return APSEP_OK;
+            return APSEP_ERR_LAUNCH;
+        }
+        finish(s, s->batch[oldest]);
+    }
+    for (int i = 0; i < SLOTS; i++) {
+        Batch& f = s->batch[i];
+        if (!f.busy) continue;
+        const hipError_t e = hipEventQuery(f.done);
+        if (e == hipErrorNotReady) {
+            (void)hipGetLastError();  // not a failure
+            continue;
         }
         if (e != hipSuccess) {
             APSEP_CLEAR_HIP_ERROR();
             return APSEP_ERR_LAUNCH;
         }
-        const double t = now_us();
-        for (int64_t id = f.first; id < f.first + f.count; id++) {
-            Job* j = job_of(s, id);
-            if (j && j->state == JOB_RUNNING) {
-                j->state = JOB_DONE;
-                j->t_done = t;
-            }
-        }
-        f.busy = false;
-        s->inf_head = (s->inf_head + 1) % INFLIGHT;
-        s->inf_count--;
-        wait = false;  // block for at most one batch
+        finish(s, f);
     }
     return APSEP_OK;
 }
 
-// pack the next run of queued jobs into one segmented solve
+// Pack the next queued jobs (in submission order) into one in-place batch on
+// the next stream with a free slot.
 static apsep_status launch_batch(apsep_server* s) {
     while (s->next_launch < s->next_id && job_of(s, s->next_launch)->n == 0)
         s->next_launch++;  // empty jobs are done at submission
-    if (s->next_launch >= s->next_id || s->inf_count >= INFLIGHT) return APSEP_OK;
-    Inflight& f = s->inf[(s->inf_head + s->inf_count) % INFLIGHT];
+    if (s->next_launch >= s->next_id) return APSEP_OK;
+    int k = -1;
+    for (int i = 0; i < NSTREAMS && k < 0; i++) {
+        const int c = (s->next_stream + i) % NSTREAMS;
+        if (s->inflight[c] < INFLIGHT) k = c;
+    }
+    if (k < 0) return APSEP_OK;
+    int slot = 0;
+    while (s->batch[slot].busy) slot++;
+    Batch& f = s->batch[slot];
+    int32_t te = 0;
+    int64_t tile_cap = 0;
+    if (apsep_batch_plan_tile_elems(s->plan[k], &te) != APSEP_OK ||
+        apsep_batch_plan_job_tiles(s->plan[k], &tile_cap) != APSEP_OK)
+        return APSEP_ERR_INVALID_ARG;
     const int64_t first = s->next_launch;
-    const Job* j0 = job_of(s, first);
-    const int64_t start = j0->slot;
-    int64_t end = start;  // end of the last job's slot
+    int64_t keys = 0, tiles = 0;
     int32_t m = 0;
     for (int64_t id = first; id < s->next_id && m < s->max_segs; id++) {
         const Job* j = job_of(s, id);
-        if (j->slot != end && m > 0) break;  // wrapped: next batch
-        if (j->slot + j->n - start > s->max_batch) break;  // the first job always fits
-        f.h_off[m] = j->slot - start;
-        end = j->slot + j->span;
+        const int64_t jt = (j->n + te - 1) / te;
+        if (m > 0 && (keys + j->n > s->max_batch || tiles + jt > tile_cap)) break;
+        f.h_jobs[m] = apsep_job{j->src, s->ring_out + j->slot, j->n, tiles};
+        keys += j->n;
+        tiles += jt;
         m++;
     }
-    const Job* jl = job_of(s, first + m - 1);
-    const int64_t total = jl->slot + jl->n - start;
-    f.h_off[m] = total;
-    uint64_t* h_src = reinterpret_cast<uint64_t*>(f.h_off + m + 1);
-    int64_t* h_len = f.h_off + 2 * m + 1;
-    for (int32_t k = 0; k < m; k++) {
-        Job* j = job_of(s, first + k);
-        h_src[k] = (uint64_t)(uintptr_t)j->src;
-        h_len[k] = j->n;
+    for (int32_t i = 0; i < m; i++) {
+        Job* j = job_of(s, first + i);
         if (j->n > 0) j->state = JOB_RUNNING;
     }
-    if (hipMemcpyAsync(f.d_off, f.h_off, (size_t)(3 * m + 1) * 8, hipMemcpyHostToDevice,
-                       s->stream) != hipSuccess) {
+    if (hipMemcpyAsync(f.d_jobs, f.h_jobs, (size_t)m * sizeof(apsep_job), hipMemcpyHostToDevice,
+                       s->stream[k]) != hipSuccess) {
         APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_LAUNCH;
     }
-    const long long nblk = (total + GATHER - 1) / GATHER;
-    hipLaunchKernelGGL(k_gather, dim3((unsigned)nblk), dim3(256), 0, s->stream,
-                       (const long long*)f.d_off, (const unsigned long long*)(f.d_off + m + 1),
-                       (const long long*)(f.d_off + 2 * m + 1), (int)m, s->ring_in + start,
-                       (long long)total);
-    if (APSEP_LAUNCH_FAILED()) return APSEP_ERR_LAUNCH;
-    apsep_status st = apsep_run_segmented(s->plan, s->ring_in + start, f.d_off,
-                                          s->ring_out + start, m, total, s->stream);
+    apsep_status st = apsep_run_jobs(s->plan[k], f.d_jobs, m, tiles, s->stream[k]);
     if (st != APSEP_OK) return st;
-    if (hipEventRecord(f.done, s->stream) != hipSuccess) {
+    if (hipEventRecord(f.done, s->stream[k]) != hipSuccess) {
         APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_LAUNCH;
     }
     f.busy = true;
+    f.stream = k;
     f.first = first;
     f.count = m;
-    s->inf_count++;
+    s->inflight[k]++;
+    s->busy++;
+    s->next_stream = (k + 1) % NSTREAMS;
     s->next_launch = first + m;
     return APSEP_OK;
 }
@@ -236,33 +233,44 @@ apsep_status apsep_server_create(int64_t pool_bytes, int64_t max_batch, int32_t 
     apsep_server* s = new (std::nothrow) apsep_server();
     if (!s) return APSEP_ERR_ALLOC;
     s->incl = incl;
-    s->cap = (pool_bytes / 8) & ~(int64_t)3;  // keys + answers, 4 B each
+    s->cap = (pool_bytes / 4) & ~(int64_t)3;  // answers, 4 B each
     s->max_batch = max_batch < s->cap ? max_batch : s->cap;
     s->max_segs = max_batch_jobs;
-    if (hipGetDevice(&s->device) != hipSuccess ||
-        hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipGetDevice(&s->device) != hipSuccess) {
         APSEP_CLEAR_HIP_ERROR();
         destroy_server(s);
         return APSEP_ERR_DEVICE;
     }
-    if (hipMalloc((void**)&s->ring_in, (size_t)s->cap * 8) != hipSuccess) {
+    for (int k = 0; k < NSTREAMS; k++)
+        if (hipStreamCreateWithFlags(&s->stream[k], hipStreamNonBlocking) != hipSuccess) {
+            APSEP_CLEAR_HIP_ERROR();
+            s->stream[k] = nullptr;
+            destroy_server(s);
+            return APSEP_ERR_DEVICE;
+        }
+    if (hipMalloc((void**)&s->ring_out, (size_t)s->cap * 4) != hipSuccess) {
         APSEP_CLEAR_HIP_ERROR();
-        s->ring_in = nullptr;
+        s->ring_out = nullptr;
         destroy_server(s);
         return APSEP_ERR_ALLOC;
     }
-    s->ring_out = s->ring_in + s->cap;
-    apsep_status st = apsep_batch_plan_create(s->max_batch, max_batch_jobs, incl, &s->plan);
-    if (st != APSEP_OK) {
-        s->plan = nullptr;
-        destroy_server(s);
-        return st;
+    // a batch of max_batch keys in max_batch_jobs jobs spans at most
+    // max_batch / tile + max_batch_jobs tiles of either shape
+    int64_t plan_cap = s->max_batch + (int64_t)max_batch_jobs * MAX_TILE;
+    if (plan_cap > INT32_MAX) plan_cap = INT32_MAX;
+    for (int k = 0; k < NSTREAMS; k++) {
+        const apsep_status st = apsep_batch_plan_create(plan_cap, max_batch_jobs, incl, &s->plan[k]);
+        if (st != APSEP_OK) {
+            s->plan[k] = nullptr;
+            destroy_server(s);
+            return st;
+        }
     }
-    for (int i = 0; i < INFLIGHT; i++) {
-        Inflight& f = s->inf[i];
+    for (int i = 0; i < SLOTS; i++) {
+        Batch& f = s->batch[i];
         if (hipEventCreateWithFlags(&f.done, hipEventDisableTiming) != hipSuccess ||
-            hipHostMalloc((void**)&f.h_off, (size_t)(3 * max_batch_jobs + 1) * 8, 0) != hipSuccess ||
-            hipMalloc((void**)&f.d_off, (size_t)(3 * max_batch_jobs + 1) * 8) != hipSuccess) {
+            hipHostMalloc((void**)&f.h_jobs, (size_t)max_batch_jobs * sizeof(apsep_job), 0) != hipSuccess ||
+            hipMalloc((void**)&f.d_jobs, (size_t)max_batch_jobs * sizeof(apsep_job)) != hipSuccess) {
             APSEP_CLEAR_HIP_ERROR();
             destroy_server(s);
             return APSEP_ERR_ALLOC;
@@ -274,7 +282,8 @@ apsep_status apsep_server_create(int64_t pool_bytes, int64_t max_batch, int32_t 
 
 apsep_status apsep_server_destroy(apsep_server* s) {
     if (!s) return APSEP_OK;
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (int k = 0; k < NSTREAMS; k++)
+        if (s->stream[k]) (void)hipStreamSynchronize(s->stream[k]);
     destroy_server(s);
     return APSEP_OK;
 }
@@ -282,6 +291,7 @@ apsep_status apsep_server_destroy(apsep_server* s) {
 apsep_status apsep_server_submit(apsep_server* s, const int32_t* d_keys, int64_t n,
                                  int64_t* job_id) {
     if (!s || !job_id || n < 0 || (n > 0 && !d_keys)) return APSEP_ERR_INVALID_ARG;
+    if (n > 0 && ((uintptr_t)d_keys & 15)) return APSEP_ERR_ALIGN;
     if (n > s->max_batch) return APSEP_ERR_CAPACITY;
     const int64_t span = (n + 3) & ~(int64_t)3;
     int64_t slot = 0;
@@ -312,11 +322,11 @@ apsep_status apsep_server_poll(apsep_server* s, int32_t* batches_launched) {
     apsep_status st = retire(s, false);
     if (st != APSEP_OK) return st;
     int32_t launched = 0;
-    while (s->inf_count < INFLIGHT && s->next_launch < s->next_id) {
-        const int before = s->inf_count;
+    while (s->busy < SLOTS && s->next_launch < s->next_id) {
+        const int before = s->busy;
         st = launch_batch(s);
         if (st != APSEP_OK) return st;
-        if (s->inf_count == before) break;
+        if (s->busy == before) break;
         launched++;
     }
     if (batches_launched) *batches_launched = launched;
@@ -403,7 +413,7 @@ apsep_status apsep_server_drain(apsep_server* s) {
     for (;;) {
         apsep_status st = apsep_server_poll(s, nullptr);
         if (st != APSEP_OK) return st;
-        if (s->inf_count == 0 && s->next_launch >= s->next_id) return APSEP_OK;
+        if (s->busy == 0 && s->next_launch >= s->next_id) return APSEP_OK;
         st = retire(s, true);
         if (st != APSEP_OK) return st;
     }

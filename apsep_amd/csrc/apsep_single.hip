@@ -67,7 +67,7 @@ __global__ __launch_bounds__(FIX_WG) __attribute__((amdgpu_waves_per_eu(8))) voi
     __shared__ int lds[FIX_WG / WAVE][FIX_LDS_INTS];
     const int t = (int)(blockIdx.x * (FIX_WG / WAVE) + (threadIdx.x >> 6));
     if (t >= L.nt) return;
-    fix_tile<INCL>(in, out, n, t, L, to, lds[threadIdx.x >> 6]);
+    fix_tile<INCL>(FlatMap{in, out}, n, t, L, to, lds[threadIdx.x >> 6]);
 }
 
 // SPT: one persistent launch of fully co-resident workgroups (launched
@@ -112,7 +112,7 @@ void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
     for (int k = wave;; k += NWAVE) {
         const long long t = (long long)blockIdx.x + (long long)k * grid;
         if (t >= nt) break;  // uniform per wave
-        fix_tile<INCL, false, true>(in, out, n, (int)t, L, to, lds);
+        fix_tile<INCL, false, true>(FlatMap{in, out}, n, (int)t, L, to, lds);
     }
 }
 

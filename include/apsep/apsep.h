@@ -216,18 +216,54 @@ apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
                                    int64_t total, void* stream, int32_t reps,
                                    float* eager_ms, float* graph_ms);
 
+/* In-place job batches: many independent arrays solved by one launch
+ * sequence, each read where it lies and answered into a buffer of the
+ * caller's (no concatenation, no copy).  No counterpart in the reference
+ * (its callers run one launchWSTL / launchSPT per array,
+ * src/apsep.cuh:2598-2605, 3069-3076).  d_jobs is a device array of
+ * `count` apsep_job; job k covers the plan's tiles
+ * first_tile .. first_tile + ceil(n / tile_elems) - 1, where tile_elems
+ * comes from apsep_batch_plan_tile_elems for the plan's current tile shape,
+ * and `tiles` is the total (first_tile of job k = the sum of the earlier
+ * jobs' tile counts).  d_answers[i] = index of the nearest qualifying
+ * predecessor of key i inside the job, or -1.  Async on `stream`; with
+ * graphs on (apsep_batch_plan_use_graph) the sequence is recorded once and
+ * replayed for the same d_jobs, count, tiles and stream (the table's
+ * contents may change).  count <= the plan's nseg_cap and tiles <=
+ * apsep_batch_plan_job_tiles, else APSEP_ERR_CAPACITY.  A tile that no job
+ * covers (tiles above the jobs' total) is solved as empty; keys and answers
+ * must not overlap.  A table whose first_tile values are not those prefix
+ * sums gives unspecified answers and may address memory outside the jobs'
+ * buffers. */
+typedef struct apsep_job {
+    const int32_t* d_keys;    /* n keys, 16-byte aligned                     */
+    int32_t* d_answers;       /* n answers, 16-byte aligned                  */
+    int64_t n;
+    int64_t first_tile;
+} apsep_job;
+
+apsep_status apsep_batch_plan_tile_elems(const apsep_batch_plan* p, int32_t* elems);
+apsep_status apsep_batch_plan_job_tiles(const apsep_batch_plan* p, int64_t* tiles);
+apsep_status apsep_run_jobs(apsep_batch_plan* p, const apsep_job* d_jobs, int32_t count,
+                            int64_t tiles, void* stream);
+
 /* ---- Serving: HBM pool, budget admission, continuous batching ----------
  * No counterpart in the reference (its launch* allocate and synchronise per
  * call, src/apsep.cuh:2598-2605, 3069-3076); for callers that solve a
  * stream of independent arrays.  A server owns a pool of pool_bytes of
- * device memory (half keys, half answers), its own stream and a batch plan;
- * one thread drives it.
- *   submit   admits a job of n keys if its slot fits in the free part of
- *            the pool (APSEP_ERR_BUDGET otherwise: nothing is queued); the
- *            device keys must stay valid until the job is done
+ * device memory for the answers, two streams and a batch plan per stream;
+ * one thread drives it.  Keys are read in place from the caller's device
+ * memory (never copied into the pool).
+ *   submit   admits a job of n keys if its answer slot fits in the free
+ *            part of the pool (APSEP_ERR_BUDGET otherwise: nothing is
+ *            queued); the device keys (16-byte aligned, APSEP_ERR_ALIGN
+ *            otherwise) must be complete when submit is called and stay
+ *            valid until the job is done
  *   poll     non-blocking: retires finished batches, then packs queued jobs
- *            (up to max_batch keys / max_batch_jobs jobs, at most two
- *            batches in flight) into segmented solves
+ *            (up to max_batch keys / max_batch_jobs jobs per batch, two
+ *            batches queued per stream) into in-place job batches
+ *            (apsep_run_jobs), alternating streams so that consecutive
+ *            batches overlap on the device
  *   job_info state (0 queued, 1 running, 2 done), submit/done times in
  *            microseconds of a monotonic clock, and the device answers
  *            (valid until release)
