@@ -1,12 +1,14 @@
 """Repeated solves of one input class, for profiling a single class.
 
-  python measure/solve_class.py KIND [--n N] [--reps R] [--algo ALGO]
+  python measure/solve_class.py KIND [--n N] [--reps R] [--algo ALGO] [--copy C]
 
 KIND is random (uniform keys in [0, 2^31 - 1), device generator seeded
 2024), descending (a[i] = N - i) or ascending (a[i] = i).  The keys are
 generated on the device, then R solves run back to back on one scratch.
 Under ``rocprofv3 --pmc`` every dispatch of the solver's kernels then
-belongs to that class.
+belongs to that class.  --copy C adds C plain device copies of N int32 (a
+kernel of known traffic, 4N bytes each way) for scaling FETCH_SIZE /
+WRITE_SIZE (measure/rocprof_summary.py traffic).
 """
 from __future__ import annotations
 
@@ -25,6 +27,7 @@ def main(argv=None) -> int:
     ap.add_argument("--n", type=int, default=1 << 25)
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--algo", default="wstl")
+    ap.add_argument("--copy", type=int, default=0)
     a = ap.parse_args(argv)
     import torch
 
@@ -39,6 +42,8 @@ def main(argv=None) -> int:
     else:
         x = torch.arange(n, dtype=torch.int32, device="cuda")
     out = torch.empty(n, dtype=torch.int32, device="cuda")
+    for _ in range(a.copy):
+        out.copy_(x)
     with A.Scratch(a.algo, n) as s:
         for _ in range(a.reps):
             s.run(x, out)
