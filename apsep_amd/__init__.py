@@ -40,6 +40,7 @@ EXPORTS = [
     "apsep_batch_plan_use_graph", "apsep_batch_plan_tune",
     "apsep_batch_plan_set_tile_lanes", "apsep_batch_plan_tile_lanes",
     "apsep_batch_plan_tile_elems", "apsep_batch_plan_job_tiles", "apsep_run_jobs",
+    "apsep_batch_plan_tuned",
     "apsep_server_create", "apsep_server_destroy", "apsep_server_submit",
     "apsep_server_poll", "apsep_server_job_info", "apsep_server_release",
     "apsep_server_drain", "apsep_server_submit_many", "apsep_server_done_upto",
@@ -118,6 +119,8 @@ def lib() -> ctypes.CDLL:
         L.apsep_batch_plan_tile_elems.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
         L.apsep_batch_plan_job_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         L.apsep_run_jobs.argtypes = [vp, vp, i32, i64, vp]
+        L.apsep_batch_plan_tuned.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                             ctypes.POINTER(ctypes.c_float)]
         L.apsep_batch_plan_tune.argtypes = [vp, vp, vp, vp, i32, i64, vp, i32,
                                             ctypes.POINTER(ctypes.c_float),
                                             ctypes.POINTER(ctypes.c_float)]
@@ -318,8 +321,17 @@ class BatchPlan:
         _check(lib().apsep_batch_plan_use_graph(self._h, int(enable)),
                "apsep_batch_plan_use_graph")
 
+    def tuned(self, size_class: int):
+        """(tile lanes, graph, ms) the tuner recorded for batches of
+        2^(c-1) + 1 .. 2^c keys; lanes = 0 when the class is untuned."""
+        ln, g, ms = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_float(0)
+        _check(lib().apsep_batch_plan_tuned(self._h, size_class, ctypes.byref(ln),
+                                            ctypes.byref(g), ctypes.byref(ms)),
+               "apsep_batch_plan_tuned")
+        return ln.value, g.value, ms.value
+
     def set_tile_lanes(self, lanes: int):
-        """Tile shape of this plan's batches: 256 or 512 lanes x 16 keys."""
+        """Tile shape of this plan's batches: 128, 256 or 512 lanes x 16 keys."""
         _check(lib().apsep_batch_plan_set_tile_lanes(self._h, int(lanes)),
                "apsep_batch_plan_set_tile_lanes")
 

@@ -14,7 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libapsep.so")
 
-SOURCES = ["apsep_single.hip", "apsep_segmented_256.hip", "apsep_segmented_512.hip", "apsep_lookback.hip", "apsep_server.hip", "apsep_api.hip"]
+SOURCES = ["apsep_single.hip", "apsep_segmented_128.hip", "apsep_segmented_256.hip",
+           "apsep_segmented_512.hip", "apsep_lookback.hip", "apsep_server.hip", "apsep_api.hip"]
 HEADERS = ["apsep_device.hpp", "apsep_internal.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

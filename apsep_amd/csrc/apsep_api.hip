@@ -260,10 +260,10 @@ apsep_status apsep_batch_plan_create(int64_t total_cap, int32_t nseg_cap, int in
     if (!p) return APSEP_ERR_ALLOC;
     p->total_cap = total_cap;
     p->nseg_cap = nseg_cap;
-    // sized for the smaller tile, which covers both shapes: as many tile
+    // sized for the smallest tile, which covers every shape: as many tile
     // slots or more, the same chunk-min count (one per 64 keys)
-    const int tile = seg256::seg_tile_elems();
-    p->tile_lanes = 256;  // the faster shape on long batches (tune picks per batch)
+    const int tile = seg128::seg_tile_elems();
+    p->tile_lanes = 256;  // until a batch picks its shape by size (or the caller / tuner does)
     apsep_status st = scratch_create(APSEP_ALGO_WSTL, total_cap, incl, tile, &p->s);
     if (st != APSEP_OK) {
         delete p;
@@ -304,14 +304,16 @@ apsep_status apsep_batch_plan_destroy(apsep_batch_plan* p) {
 
 static apsep_status plan_run(apsep_batch_plan* p, const int32_t* d_in, const int64_t* offs,
                              int32_t* d_out, int nseg, int total, hipStream_t st) {
-    return p->tile_lanes == 512 ? seg512::seg_run(p, d_in, offs, d_out, nseg, total, st)
-                                : seg256::seg_run(p, d_in, offs, d_out, nseg, total, st);
+    return p->tile_lanes == 512   ? seg512::seg_run(p, d_in, offs, d_out, nseg, total, st)
+           : p->tile_lanes == 128 ? seg128::seg_run(p, d_in, offs, d_out, nseg, total, st)
+                                  : seg256::seg_run(p, d_in, offs, d_out, nseg, total, st);
 }
 
 static apsep_status plan_run_jobs(apsep_batch_plan* p, const apsep_job* jobs, int count,
                                   int tiles, hipStream_t st) {
-    return p->tile_lanes == 512 ? seg512::jobs_run(p, jobs, count, tiles, st)
-                                : seg256::jobs_run(p, jobs, count, tiles, st);
+    return p->tile_lanes == 512   ? seg512::jobs_run(p, jobs, count, tiles, st)
+           : p->tile_lanes == 128 ? seg128::jobs_run(p, jobs, count, tiles, st)
+                                  : seg256::jobs_run(p, jobs, count, tiles, st);
 }
 
 // Replay the plan's graph when it was recorded for the same call (kind 0:
@@ -361,10 +363,16 @@ static apsep_status graph_run(apsep_batch_plan* p, int kind, const void* a, cons
     return APSEP_OK;
 }
 
-apsep_status apsep_batch_plan_set_tile_lanes(apsep_batch_plan* p, int32_t lanes) {
-    if (!p || (lanes != 256 && lanes != 512)) return APSEP_ERR_INVALID_ARG;
+static void set_lanes(apsep_batch_plan* p, int lanes) {
     if (lanes != p->tile_lanes) drop_graph(p);  // a recorded graph holds the other shape
     p->tile_lanes = lanes;
+}
+
+apsep_status apsep_batch_plan_set_tile_lanes(apsep_batch_plan* p, int32_t lanes) {
+    if (!p || (lanes != 128 && lanes != 256 && lanes != 512)) return APSEP_ERR_INVALID_ARG;
+    set_lanes(p, lanes);
+    p->tuned_auto = 0;  // an explicit shape overrides the tuned table
+    p->lanes_explicit = 1;
     return APSEP_OK;
 }
 
@@ -375,7 +383,24 @@ apsep_status apsep_batch_plan_tile_lanes(const apsep_batch_plan* p, int32_t* lan
 }
 
 static int plan_tile_elems(const apsep_batch_plan* p) {
-    return p->tile_lanes == 512 ? seg512::seg_tile_elems() : seg256::seg_tile_elems();
+    return p->tile_lanes == 512   ? seg512::seg_tile_elems()
+           : p->tile_lanes == 128 ? seg128::seg_tile_elems()
+                                  : seg256::seg_tile_elems();
+}
+
+// Size class of a batch of `total` keys: c with 2^(c-1) < total <= 2^c.
+static int size_class(int64_t total) {
+    return total <= 1 ? 0 : 64 - __builtin_clzll((unsigned long long)(total - 1));
+}
+
+apsep_status apsep_batch_plan_tuned(const apsep_batch_plan* p, int32_t size_class_log2,
+                                    int32_t* lanes, int32_t* graph, float* ms) {
+    if (!p || !lanes || !graph || size_class_log2 < 0 || size_class_log2 > 31)
+        return APSEP_ERR_INVALID_ARG;
+    *lanes = p->cls_lanes[size_class_log2];
+    *graph = p->cls_graph[size_class_log2];
+    if (ms) *ms = p->cls_ms[size_class_log2];
+    return APSEP_OK;
 }
 
 apsep_status apsep_batch_plan_tile_elems(const apsep_batch_plan* p, int32_t* elems) {
@@ -401,6 +426,7 @@ apsep_status apsep_batch_plan_use_graph(apsep_batch_plan* p, int enable) {
     if (!p || (enable != 0 && enable != 1)) return APSEP_ERR_INVALID_ARG;
     if (!enable) drop_graph(p);
     p->use_graph = enable;
+    p->tuned_auto = 0;  // an explicit mode overrides the tuned table
     return APSEP_OK;
 }
 
@@ -421,6 +447,15 @@ apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
         return APSEP_ERR_DEVICE;
     }
     hipStream_t hs = (hipStream_t)stream;
+    const int c = size_class(total);
+    if (p->tuned_auto && p->cls_lanes[c]) {
+        // the tuned shape and launch mode of this batch's size class
+        set_lanes(p, p->cls_lanes[c]);
+        if (!p->cls_graph[c]) drop_graph(p);
+        p->use_graph = p->cls_graph[c];
+    } else if (!p->lanes_explicit) {
+        set_lanes(p, default_batch_lanes(total));
+    }
     if (!p->use_graph || !stream)
         return plan_run(p, d_in, d_offsets, d_out, nseg, (int)total, hs);
     return graph_run(p, 0, d_in, d_offsets, d_out, nseg, total, stream);
@@ -460,11 +495,13 @@ apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
         (void)hipEventDestroy(e0);
         return APSEP_ERR_DEVICE;
     }
+    const int explicit_before = p->lanes_explicit;
     // every tile shape x launch mode, timed on this batch
-    const int32_t shapes[2] = {256, 512};
-    float t[2][2] = {{0.f, 0.f}, {0.f, 0.f}};
+    constexpr int NSHAPES = 3;
+    const int32_t shapes[NSHAPES] = {128, 256, 512};
+    float t[NSHAPES][2] = {};
     apsep_status st = APSEP_OK;
-    for (int k = 0; k < 2 && st == APSEP_OK; k++) {
+    for (int k = 0; k < NSHAPES && st == APSEP_OK; k++) {
         st = apsep_batch_plan_set_tile_lanes(p, shapes[k]);
         for (int g = 0; g < 2 && st == APSEP_OK; g++) {
             st = apsep_batch_plan_use_graph(p, g);
@@ -487,7 +524,7 @@ apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
         return st;
     }
     int bk = 0, bg = 0;
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < NSHAPES; k++)
         for (int g = 0; g < 2; g++)
             if (t[k][g] < t[bk][bg]) bk = k, bg = g;
     if (eager_ms) *eager_ms = t[bk][0] / reps;
@@ -495,7 +532,17 @@ apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
     st = apsep_batch_plan_set_tile_lanes(p, shapes[bk]);
     if (st != APSEP_OK) return st;
     // d_out holds the last shape's answer: bit-identical across shapes
-    return apsep_batch_plan_use_graph(p, bg);
+    st = apsep_batch_plan_use_graph(p, bg);
+    if (st != APSEP_OK) return st;
+    // remembered for the batch's size class; later batches of a tuned class
+    // run with its settings
+    const int c = size_class(total);
+    p->cls_lanes[c] = shapes[bk];
+    p->cls_graph[c] = bg;
+    p->cls_ms[c] = t[bk][bg] / reps;
+    p->tuned_auto = 1;
+    p->lanes_explicit = explicit_before;
+    return APSEP_OK;
 }
 
 apsep_status apsep_launch(apsep_algo algo, const int32_t* d_in, int32_t* d_out,

@@ -75,7 +75,7 @@ constexpr int QCAP = 192;          // per-wave LDS queue of cross-lane queries
 constexpr int XQ = 3;              // further queries per lane tried on the previous lane
 // segmented solves try them only with 256-lane tiles (run at 5 waves per
 // SIMD, room to spare); 512-lane segmented tiles run at 6 and would spill
-constexpr int XQ_SEG = NT == 256 ? XQ : 0;
+constexpr int XQ_SEG = NT <= 256 ? XQ : 0;
 #ifndef APSEP_XQ_SPT
 #define APSEP_XQ_SPT 2
 #endif
@@ -91,7 +91,7 @@ constexpr int INF = INT_MAX;
 #define SPT_WAVES_PER_EU 6         // occupancy the persistent kernel is pinned at
 
 static_assert(K == 16 && LPC == 4 && (1 << LOG_LPC) == LPC, "tile shape");
-static_assert(NWAVE == 4 || NWAVE == 8, "the wave searches handle 4 or 8 waves per tile");
+static_assert(NWAVE == 2 || NWAVE == 4 || NWAVE == 8, "the wave searches handle 2, 4 or 8 waves per tile");
 static_assert(CPT % 16 == 0, "chunk-min search works in blocks of 16");
 
 typedef unsigned long long u64;
@@ -667,8 +667,10 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     int cross = INF, after = INF;
     {
         int wm[8];
+        // (read in groups of four: with two waves the second half of the
+        // group lies past wmin and is not used)
 #pragma unroll
-        for (int h = 0; h < NWAVE / 4; h++) {
+        for (int h = 0; h < (NWAVE + 3) / 4; h++) {
             const int4 a = *reinterpret_cast<const int4*>(&sm.wmin[4 * h]);
             wm[4 * h] = __builtin_amdgcn_readfirstlane(a.x);
             wm[4 * h + 1] = __builtin_amdgcn_readfirstlane(a.y);

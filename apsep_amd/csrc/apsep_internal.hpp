@@ -45,8 +45,15 @@ struct apsep_batch_plan {
     int* tmap;             // [tiles at capacity]
     int2* tbound;          // [tiles at capacity]: that segment's start and end
     apsep::Page* pages;    // [tiles at capacity]: in-place job batches
-    int tile_lanes;        // tile shape the batches run with: 256 or 512 lanes
+    int tile_lanes;        // tile shape the batches run with: 128, 256 or 512 lanes
     int use_graph;
+    // autotuned settings per size class (class c: batches of 2^(c-1) + 1 ..
+    // 2^c keys), applied to apsep_run_segmented while `tuned_auto` is set
+    int tuned_auto;
+    int lanes_explicit;    // the caller chose the tile shape (no size rule)
+    int cls_lanes[32];     // 0: class not tuned
+    int cls_graph[32];
+    float cls_ms[32];
     hipGraph_t graph;
     hipGraphExec_t exec;
     // arguments the captured graph was recorded with
@@ -60,6 +67,11 @@ struct apsep_batch_plan {
 };
 
 namespace apsep {
+
+// Tile lanes for a batch of `total` keys when neither the caller nor the
+// tuner chose (measured on MI355X: 128-lane tiles are fastest up to 2^26
+// keys, where a batch does not fill the GPU with 256-lane tiles; 256 beyond).
+inline int default_batch_lanes(int64_t total) { return total <= (1ll << 26) ? 128 : 256; }
 
 // The HIP error behind the calling thread's last failing status: recorded
 // where the library clears HIP's sticky error (apsep_last_error reports it).
@@ -85,9 +97,16 @@ apsep_status single_run(apsep_scratch* s, const int32_t* in, int32_t* out, int n
 apsep_status single_error(apsep_scratch* s, int* code);
 // apsep_run_noalloc's solve: scratch in a per-device file-scope array
 apsep_status noalloc_run(const int32_t* in, int32_t* out, int n, int incl, hipStream_t st);
-// The segmented solver, built for tiles of 256 and of 512 lanes
-// (apsep_segmented_256.hip, apsep_segmented_512.hip).  seg_tile_elems is
-// that build's tile in elements.
+// The segmented solver, built for tiles of 128, 256 and 512 lanes
+// (apsep_segmented_128.hip, _256.hip, _512.hip).  seg_tile_elems is that
+// build's tile in elements.
+namespace seg128 {
+apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs,
+                     int32_t* out, int nseg, int n, hipStream_t st);
+apsep_status jobs_run(apsep_batch_plan* p, const void* jobs, int njobs, int tiles,
+                      hipStream_t st);
+int seg_tile_elems();
+}  // namespace seg128
 namespace seg256 {
 apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs,
                      int32_t* out, int nseg, int n, hipStream_t st);

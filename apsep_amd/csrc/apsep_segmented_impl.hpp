@@ -71,9 +71,10 @@ __global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__
 
 // 256-lane tiles: 5 waves per SIMD (88 VGPRs, no spill; 5 tiles per CU)
 // beat 6 (80 VGPRs and a 16-byte spill): 1.47 instead of 1.53 ms for the
-// 8192-segment batch.  512-lane tiles keep 6 (3 tiles per CU, not 2).
+// 8192-segment batch.  128-lane tiles likewise run at 5 (at 6 they spill 35
+// registers).  512-lane tiles keep 6 (3 tiles per CU, not 2).
 template <bool INCL>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 256 ? 5 : 6))) void k_tile_pass_seg(const int* __restrict__ in,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT <= 256 ? 5 : 6))) void k_tile_pass_seg(const int* __restrict__ in,
                                                       int* __restrict__ out, int n,
                                                       TileOut to, Levels L, SegIn sg) {
     __shared__ TileSmem sm;
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(256) void k_jobs_setup(const JobDesc* __restrict__ 
 }
 
 template <bool INCL>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 256 ? 5 : 6))) void k_tile_pass_jobs(const Page* __restrict__ pages,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT <= 256 ? 5 : 6))) void k_tile_pass_jobs(const Page* __restrict__ pages,
                                                        const int2* __restrict__ tbound,
                                                        TileOut to, Levels L) {
     __shared__ TileSmem sm;

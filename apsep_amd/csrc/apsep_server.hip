@@ -178,15 +178,27 @@ static apsep_status launch_batch(apsep_server* s) {
     int slot = 0;
     while (s->batch[slot].busy) slot++;
     Batch& f = s->batch[slot];
-    int32_t te = 0;
-    int64_t tile_cap = 0;
-    if (apsep_batch_plan_tile_elems(s->plan[k], &te) != APSEP_OK ||
-        apsep_batch_plan_job_tiles(s->plan[k], &tile_cap) != APSEP_OK)
-        return APSEP_ERR_INVALID_ARG;
     const int64_t first = s->next_launch;
-    int64_t keys = 0, tiles = 0;
+    // the batch's keys, then the tile shape for that size
+    int64_t keys = 0;
     int32_t m = 0;
     for (int64_t id = first; id < s->next_id && m < s->max_segs; id++) {
+        const Job* j = job_of(s, id);
+        if (m > 0 && keys + j->n > s->max_batch) break;
+        keys += j->n;
+        m++;
+    }
+    int32_t te = 0;
+    int64_t tile_cap = 0;
+    if (apsep_batch_plan_set_tile_lanes(s->plan[k], apsep::default_batch_lanes(keys)) != APSEP_OK ||
+        apsep_batch_plan_tile_elems(s->plan[k], &te) != APSEP_OK ||
+        apsep_batch_plan_job_tiles(s->plan[k], &tile_cap) != APSEP_OK)
+        return APSEP_ERR_INVALID_ARG;
+    const int32_t m_max = m;
+    int64_t tiles = 0;
+    keys = 0;
+    m = 0;
+    for (int64_t id = first; id < s->next_id && m < m_max; id++) {
         const Job* j = job_of(s, id);
         const int64_t jt = (j->n + te - 1) / te;
         if (m > 0 && (keys + j->n > s->max_batch || tiles + jt > tile_cap)) break;

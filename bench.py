@@ -372,6 +372,9 @@ def run_serve(args, torch, apsep_amd, cpu: bool):
     sample_answers = {}
     ptr_arr = np.array([base + 4 * int(off[k]) for k in range(P)], dtype=np.uint64)
     len_arr = lengths.astype(np.int64)
+    # sampled answers are copied on a stream of their own: a copy on the
+    # legacy stream would wait for every batch the server has queued
+    side = torch.cuda.Stream()
     with apsep_amd.Server(64 << 30, 1 << 28, 4096) as srv:
         for rate in rates:
             arr = (time.monotonic_ns() / 1e3 + 2e5
@@ -399,9 +402,10 @@ def run_serve(args, torch, apsep_amd, cpu: bool):
                         _, done = srv.job_times(id0 + rel, d - rel)
                         lat[rel:d] = done - arr[rel:d]
                         if rate == rates[-1] and cpu:
-                            for j in range(rel, d):
-                                if j % 97 == 0:
-                                    sample_answers[j % P] = srv.answers(id0 + j).cpu().numpy()
+                            with torch.cuda.stream(side):
+                                for j in range(rel, d):
+                                    if j % 97 == 0:
+                                        sample_answers[j % P] = srv.answers(id0 + j)
                         t_last = float(done[-1])
                         srv.release_upto(id0 + d)
                         rel = d
@@ -473,7 +477,8 @@ def run_serve(args, torch, apsep_amd, cpu: bool):
             "sample": "the %d distinct payloads once, as independent jobs on a %d-thread "
                       "pool (oracle/apsep_oracle.c -O3), %.2f s" % (P, threads, secs)}
         rec["parity_checked"] = bool(sample_answers) and bool(all(
-            np.array_equal(v, exp[hl[k]:hl[k + 1]]) for k, v in sample_answers.items()))
+            np.array_equal(v.cpu().numpy(), exp[hl[k]:hl[k + 1]])
+            for k, v in sample_answers.items()))
         rec["parity_jobs_checked"] = len(sample_answers)
     del src, outp
     torch.cuda.empty_cache()
@@ -493,6 +498,59 @@ def bench_serve(args, torch, apsep_amd):
     }
     line.update({k: v for k, v in rec.items() if k != "workload"})
     print(json.dumps(line), flush=True)
+
+
+def bench_tune(args, torch, apsep_amd):
+    """The autotuner's choices per size class: for batches of about 2^c keys
+    (c = 14 .. 28; log-uniform segment lengths 64 .. min(2^20, 2^c / 4),
+    class per segment uniform over random / descending / ascending), every
+    tile shape x launch mode timed (median of `steps` runs), and what
+    apsep_batch_plan_tune keeps for the class."""
+    import numpy as np
+    rows = []
+    stream = torch.cuda.Stream()
+    for c in range(14, 29):
+        target = 1 << c
+        hi = max(64, min(1 << 20, target >> 2))
+        mean = (hi - 64) / np.log(hi / 64.0) if hi > 64 else 64
+        nseg = max(1, min(200000, int(round(target / mean))))
+        x, off, lengths, _ = batch_inputs(torch, nseg, 64, hi, 1000 + c)
+        total = int(off[-1])
+        offs = torch.from_numpy(off).cuda()
+        out = torch.empty(total, dtype=torch.int32, device="cuda")
+        row = {"target_log2": c, "class": int(total - 1).bit_length(), "keys": total,
+               "segments": nseg, "times_ms": {}}
+        with apsep_amd.BatchPlan(total, nseg) as p, torch.cuda.stream(stream):
+            for lanes in (128, 256, 512):
+                for graph in (0, 1):
+                    p.set_tile_lanes(lanes)
+                    p.use_graph(bool(graph))
+                    run = lambda: p.run(x, offs, out, total=total, stream=stream)
+                    run()
+                    run()
+                    ts = []
+                    for _ in range(args.steps):
+                        ts.append(_timed(torch, stream, run, 1))
+                    row["times_ms"]["%d%s" % (lanes, "g" if graph else "e")] = \
+                        round(sorted(ts)[len(ts) // 2], 4)
+            p.tune(x, offs, out, total=total, stream=stream, reps=max(3, args.steps // 2))
+            cls = int(total - 1).bit_length()  # the library's size class of `total`
+            lanes, graph, ms = p.tuned(cls)
+            row["tuned"] = {"lanes": lanes, "graph": bool(graph), "ms": round(ms, 4)}
+        best = min(row["times_ms"], key=row["times_ms"].get)
+        row["best"] = best
+        row["best_GBps"] = round(8.0 * total / (row["times_ms"][best] * 1e-3) / 1e9, 1)
+        rows.append(row)
+        print("tune class %d: %s" % (c, json.dumps(row)), file=sys.stderr, flush=True)
+        del x, out, offs
+        torch.cuda.empty_cache()
+    print(json.dumps({
+        "metric": "segmented batch time per size class, every tile shape x launch mode",
+        "value": None, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 2,
+        "ms_per_step": None, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (numpy lengths/classes, device-generated keys)",
+        "config": {"workload": "autotuner size classes 2^14..2^28"}, "rows": rows}), flush=True)
 
 
 def bench_sweep(args, torch, apsep_amd):
@@ -676,11 +734,12 @@ def main():
                     help="single: leave out the batch and serve sections")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="rocprofv3 FETCH/WRITE summary (bytes per launch)")
-    ap.add_argument("--workload", default="single", choices=["single", "batch", "serve", "sweep"],
+    ap.add_argument("--workload", default="single",
+                    choices=["single", "batch", "serve", "sweep", "tune"],
                     help="single: the plain N=2^25 solve (headline) with the batch and "
                          "serve sections; batch: the segmented CSR batch alone; serve: "
                          "the serving engine alone; sweep: sizes 2^14..2^27, cache-warm "
-                         "and flushed")
+                         "and flushed; tune: the autotuner's choice per size class")
     ap.add_argument("--nseg", type=int, default=8192)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-warmup", type=int, default=3)
@@ -716,6 +775,9 @@ def main():
         return
     if args.workload == "sweep":
         bench_sweep(args, torch, apsep_amd)
+        return
+    if args.workload == "tune":
+        bench_tune(args, torch, apsep_amd)
         return
 
     rec = run_single(args, torch, apsep_amd)

@@ -200,21 +200,34 @@ apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
  * The sequence holds no memset node. */
 apsep_status apsep_batch_plan_use_graph(apsep_batch_plan* p, int enable);
 
-/* Tile shape of a plan's batches: 256 (the default) or 512 lanes of 16
+/* Tile shape of a plan's batches: 128, 256 (the default) or 512 lanes of 16
  * keys per workgroup.  Answers do not depend on it.  No counterpart in the
  * reference (its kernels take BLOCK_SIZE / IPT as template parameters,
- * src/apsep.cuh:2548, 3052); set_tile_lanes drops a recorded graph. */
+ * src/apsep.cuh:2548, 3052); set_tile_lanes drops a recorded graph, and it
+ * and apsep_batch_plan_use_graph switch the tuned table (below) off.  Until
+ * the caller sets a shape, apsep_run_segmented picks it per batch by size
+ * (128 lanes up to 2^26 keys, 256 beyond: the faster shapes measured on
+ * MI355X); apsep_run_jobs always runs the current shape, whose tile size
+ * the caller's job table is built for. */
 apsep_status apsep_batch_plan_set_tile_lanes(apsep_batch_plan* p, int32_t lanes);
 apsep_status apsep_batch_plan_tile_lanes(const apsep_batch_plan* p, int32_t* lanes);
 
-/* Autotune a plan for a batch shape: for each tile shape, times `reps`
+/* Autotune a plan for a batch: for each tile shape, times `reps`
  * direct-launch runs and `reps` graph replays of this batch on `stream`
  * (non-null); keeps the fastest shape and mode and reports that shape's two
- * mean times (blocking; d_out holds the answer afterwards). */
+ * mean times (blocking; d_out holds the answer afterwards).  The choice is
+ * also recorded for the batch's size class (c with 2^(c-1) < total <= 2^c):
+ * from then on apsep_run_segmented runs every batch of a tuned class with
+ * that class's shape and mode. */
 apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
                                    const int64_t* d_offsets, int32_t* d_out, int32_t nseg,
                                    int64_t total, void* stream, int32_t reps,
                                    float* eager_ms, float* graph_ms);
+
+/* The tuned entry of size class `size_class_log2` (0..31): tile lanes (0
+ * when the class has not been tuned), graph mode and its mean time in ms. */
+apsep_status apsep_batch_plan_tuned(const apsep_batch_plan* p, int32_t size_class_log2,
+                                    int32_t* lanes, int32_t* graph, float* ms);
 
 /* In-place job batches: many independent arrays solved by one launch
  * sequence, each read where it lies and answered into a buffer of the

@@ -60,7 +60,7 @@ def _edge_lengths(T):
             2 * T + 1, 64 * T + 5, 1, 0, 3 * T + 64, 100000, 7]
 
 
-@pytest.mark.parametrize("lanes", [256, 512])
+@pytest.mark.parametrize("lanes", [128, 256, 512])
 @pytest.mark.parametrize("incl", [False, True])
 def test_jobs_edges_and_classes(torch_mod, lanes, incl):
     import apsep_amd as A
@@ -77,7 +77,7 @@ def test_jobs_edges_and_classes(torch_mod, lanes, incl):
         _check(host, outs, incl, "edges lanes=%d incl=%d" % (lanes, incl))
 
 
-@pytest.mark.parametrize("lanes", [256, 512])
+@pytest.mark.parametrize("lanes", [128, 256, 512])
 def test_jobs_many_small_and_log_uniform(torch_mod, lanes):
     import apsep_amd as A
     torch = torch_mod

@@ -2,7 +2,7 @@
 oracle's per-segment monotone stack (answers relative to each segment
 start).  Covers empty and one-element segments, segments that start inside
 a lane, at lane/chunk/tile edges and span many tiles, every input class,
-both predicates, both tile shapes (256 and 512 lanes), the autotuner's
+both predicates, every tile shape (128, 256 and 512 lanes), the autotuner's
 choice, and hipGraph capture/replay with new data between replays.
 """
 import sys
@@ -68,7 +68,7 @@ EDGE_LENGTHS = [0, 1, 2, 15, 16, 17, 0, 63, 64, 65, 4095, 4096, 4097, 1, 0, 8191
                 3, 70000, 5, 0, 4096 * 70 + 3, 7, 1]
 
 
-@pytest.mark.parametrize("lanes", [256, 512])
+@pytest.mark.parametrize("lanes", [128, 256, 512])
 @pytest.mark.parametrize("incl", [False, True])
 @pytest.mark.parametrize("kind", ["random", "descending", "ascending", "dup2", "mixed",
                                   "intmax", "alleq"])
@@ -85,7 +85,7 @@ def test_edge_lengths(torch_mod, kind, incl, lanes):
                "edge %s incl=%d lanes=%d" % (kind, incl, lanes))
 
 
-@pytest.mark.parametrize("lanes", [256, 512])
+@pytest.mark.parametrize("lanes", [128, 256, 512])
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_many_small_segments(torch_mod, seed, lanes):
     """Thousands of segments shorter than a lane / a tile (many starts per
@@ -102,7 +102,7 @@ def test_many_small_segments(torch_mod, seed, lanes):
                "small seed=%d lanes=%d" % (seed, lanes))
 
 
-@pytest.mark.parametrize("lanes", [256, 512])
+@pytest.mark.parametrize("lanes", [128, 256, 512])
 def test_loguniform_batch(torch_mod, lanes):
     """A scaled-down batched configuration: log-uniform lengths in
     [64, 2^18], class uniform over random/descending/ascending."""
@@ -142,10 +142,10 @@ def test_tune_picks_a_shape_and_switching_shapes_keeps_answers(torch_mod):
         stream.synchronize()
         assert e > 0 and g > 0
         chosen = p.tile_lanes
-        assert chosen in (256, 512)
+        assert chosen in (128, 256, 512)
         _check(out.cpu().numpy(), a, off, False, "after tune (lanes=%d)" % chosen)
         p.use_graph(True)
-        for lanes in (512, 256, 512):
+        for lanes in (512, 128, 256, 512):
             p.set_tile_lanes(lanes)
             out.fill_(int(POISON))
             torch.cuda.synchronize()
@@ -155,7 +155,7 @@ def test_tune_picks_a_shape_and_switching_shapes_keeps_answers(torch_mod):
             _check(out.cpu().numpy(), a, off, False, "graph plan lanes=%d" % lanes)
 
 
-@pytest.mark.parametrize("lanes", [256, 512])
+@pytest.mark.parametrize("lanes", [128, 256, 512])
 @pytest.mark.parametrize("cap", [3 * 8192 + 1, 5 * 8192 + 4095, 7 * 4096 + 17])
 def test_capacity_off_tile_multiples(torch_mod, cap, lanes):
     """Plans whose capacity is not a multiple of either tile, filled to
@@ -174,7 +174,7 @@ def test_capacity_off_tile_multiples(torch_mod, cap, lanes):
                    "cap=%d lanes=%d %s" % (cap, lanes, kind))
 
 
-@pytest.mark.parametrize("lanes", [256, 512])
+@pytest.mark.parametrize("lanes", [128, 256, 512])
 def test_deep_hierarchy_batch(torch_mod, lanes):
     """2^28 + 12345 keys in 64 segments: level 1 holds more than 512 entries
     with either tile shape, so levels >= 2 are built (one launch per level)
@@ -296,3 +296,35 @@ def test_c4_full_batch_eager_and_graph(torch_mod):
             check("graph replay %d lanes=%d" % (rep, p.tile_lanes))
     del x, out
     torch.cuda.empty_cache()
+
+
+def test_tuned_size_classes(torch_mod):
+    """The autotuner records its choice per size class; later batches of a
+    tuned class run with that class's shape and mode, batches of an untuned
+    class with the plan's current settings, and every answer is exact."""
+    import apsep_amd as A
+    torch = torch_mod
+    stream = torch.cuda.Stream()
+    batches = []
+    for seed, nseg, hi in ((1, 40, 1 << 12), (2, 300, 1 << 17), (3, 200, 1 << 20)):
+        off = oracle.loguniform_offsets(seed, nseg, 64.0, float(hi))
+        a = _keys("mixed", np.diff(off), seed)
+        batches.append((a, off))
+    cap = max(a.shape[0] for a, _ in batches)
+    with A.BatchPlan(cap, 300) as p, torch.cuda.stream(stream):
+        for a, off in batches[:2]:  # tune two classes
+            x = torch.from_numpy(a).cuda()
+            o = torch.from_numpy(off).cuda()
+            out = torch.empty_like(x)
+            p.tune(x, o, out, stream=stream)
+            torch.cuda.synchronize()
+            _check(out.cpu().numpy(), a, off, False, "tune")
+        tuned = [p.tuned(c) for c in range(32)]
+        classes = [c for c in range(32) if tuned[c][0]]
+        _log("tuned classes", [(c, tuned[c]) for c in classes])
+        assert len(classes) == 2 and all(tuned[c][0] in (128, 256, 512) for c in classes)
+        for rep in range(2):
+            for i, (a, off) in enumerate(batches):  # the third class is untuned
+                _log("class run", rep, i, a.shape[0])
+                _check(_solve(p, torch, a, off, stream=stream), a, off, False,
+                       "class run %d batch %d" % (rep, i))
