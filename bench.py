@@ -86,9 +86,11 @@ def time_solves(torch, apsep_amd, scratch, xs, outs, steps, warmup, stream, per_
     """Timed region: `steps` steps of one solve per class, back to back on
     `stream` with nothing between the solves, one HIP event pair around the
     whole region.  Then, outside it: the per-class breakdown (`per_reps`
-    solves of one class back to back between one event pair), each kernel
-    of the solve launched over the three classes in turn (one scratch per
-    class), and the same kernels re-run on one class at a time (warm)."""
+    solves of one class back to back between one event pair, over three
+    distinct input/output pairs of that class in turn, and again on one pair
+    (warm)), each kernel of the solve launched over the three classes in turn
+    (one scratch per class), and the same kernels re-run on one class at a
+    time (warm)."""
     names = list(xs)
     for _ in range(warmup):
         for k in names:
@@ -103,16 +105,21 @@ def time_solves(torch, apsep_amd, scratch, xs, outs, steps, warmup, stream, per_
     t_all1.record(stream)
     torch.cuda.synchronize()
     scratch.check()
-    per = {}
+    per, per_warm = {}, {}
     for k in names:
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        for _ in range(per_reps):
-            scratch.run(xs[k], outs[k], stream=stream)
-        b.record(stream)
+        ins = [xs[k]] + [xs[k].clone() for _ in range(2)]
+        res = [outs[k]] + [torch.empty_like(outs[k]) for _ in range(2)]
         torch.cuda.synchronize()
-        per[k] = a.elapsed_time(b) / per_reps
+        for m, dst in ((3, per), (1, per_warm)):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for r in range(per_reps):
+                scratch.run(ins[r % m], res[r % m], stream=stream)
+            b.record(stream)
+            torch.cuda.synchronize()
+            dst[k] = a.elapsed_time(b) / per_reps
+        del ins, res
     scratch.check()
     n = xs[names[0]].numel()
     algo = {v: k for k, v in apsep_amd.ALGOS.items()}[scratch.algo]
@@ -128,7 +135,7 @@ def time_solves(torch, apsep_amd, scratch, xs, outs, steps, warmup, stream, per_
             r.close()
     warm = {k: scratch.time_kernels(xs[k], outs[k], per_reps, stream=stream) for k in names}
     scratch.check()
-    return per, t_all0.elapsed_time(t_all1), rot, warm
+    return per, per_warm, t_all0.elapsed_time(t_all1), rot, warm
 
 
 def cpu_baseline(torch, xs, outs, budget_s: float):
@@ -638,8 +645,9 @@ def run_single(args, torch, apsep_amd):
         t0 = time.perf_counter()
         settle(torch, s, xs, outs, stream, args.settle)
         settle_s = time.perf_counter() - t0
-        per, total_ms, rot, warm = time_solves(torch, apsep_amd, s, xs, outs, args.steps,
-                                               args.warmup, stream, per_reps)
+        per, per_warm, total_ms, rot, warm = time_solves(torch, apsep_amd, s, xs, outs,
+                                                         args.steps, args.warmup, stream,
+                                                         per_reps)
         algo_name = {v: k for k, v in apsep_amd.ALGOS.items()}[s.algo]
     useful = 8.0 * n
     per_class = {k: round(useful / (per[k] * 1e-3) / 1e9, 1) for k in CLASSES}
@@ -682,6 +690,10 @@ def run_single(args, torch, apsep_amd):
         "settle_s": round(settle_s, 2),
         "per_class_gbs": per_class,
         "per_class_us": {k: round(per[k] * 1e3, 2) for k in CLASSES},
+        "per_class_regime": "3 distinct input/output pairs of the class in turn",
+        "per_class_gbs_warm": {k: round(useful / (per_warm[k] * 1e-3) / 1e9, 1)
+                               for k in CLASSES},
+        "per_class_us_warm": {k: round(per_warm[k] * 1e3, 2) for k in CLASSES},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,

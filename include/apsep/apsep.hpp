@@ -14,6 +14,8 @@
 //     loads value-convert lanes, src/apsep.cuh:2689-2691).
 //   * BLOCK_SIZE and IPT are accepted and ignored: the gfx950 engine uses its
 //     own wave64 tile shape (512 lanes x 16 elements).
+//   * runSPT / launchSPT run the fastest solver on the device unless
+//     APSEP_SHIM_SPT_ALGO selects another (the SPT section below).
 //   * run* launch on the default stream, asynchronously, like the reference;
 //     an optional trailing stream argument (hipStream_t as void*) is added.
 //   * Errors print "APSEP error <file>:<line>: <text> (<HIP error>)" and
@@ -95,12 +97,22 @@ void launchWSTL(const T* d_in, int* d_out, int n) {
 }
 
 // ----------------------------------------------------------------- SPT
+//
+// The SPT names solve with APSEP_SHIM_SPT_ALGO, by default APSEP_ALGO_AUTO:
+// the fastest solver on the device, which on MI355X is the multi-launch
+// one (a cooperative launch alone costs ~17 us more than a normal launch,
+// more than the whole fix-up pass).  Every algorithm gives the same
+// answers.  Define APSEP_SHIM_SPT_ALGO as APSEP_ALGO_SPT before including
+// this header to run the single persistent cooperative kernel instead.
+#ifndef APSEP_SHIM_SPT_ALGO
+#define APSEP_SHIM_SPT_ALGO APSEP_ALGO_AUTO
+#endif
 
 template <typename T, int BLOCK_SIZE = 128, int IPT = 4, bool INCL = false>
 SPTScratch<T, BLOCK_SIZE, IPT> allocSPTScratch(int n) {
     static_assert(std::is_same<T, int>::value, "apsep: int32 keys only");
     SPTScratch<T, BLOCK_SIZE, IPT> s;
-    APSEP_SHIM_CHECK(apsep_scratch_create(APSEP_ALGO_SPT, n > 0 ? n : 0, INCL ? 1 : 0,
+    APSEP_SHIM_CHECK(apsep_scratch_create(APSEP_SHIM_SPT_ALGO, n > 0 ? n : 0, INCL ? 1 : 0,
                                           &s.handle));
     apsep_shim::fill(s);
     return s;
@@ -124,7 +136,7 @@ template <typename T, int BLOCK_SIZE = 128, int IPT = 4, bool INCL = false>
 void launchSPT(const T* d_in, int* d_out, int n) {
     static_assert(std::is_same<T, int>::value, "apsep: int32 keys only");
     if (n <= 0) return;
-    APSEP_SHIM_CHECK(apsep_launch(APSEP_ALGO_SPT, d_in, d_out, n, INCL ? 1 : 0));
+    APSEP_SHIM_CHECK(apsep_launch(APSEP_SHIM_SPT_ALGO, d_in, d_out, n, INCL ? 1 : 0));
 }
 
 #endif  // APSEP_APSEP_HPP

@@ -1,7 +1,9 @@
 """The drop-in path run for real, checked bit-exact against the oracle.
 
 * tests/dropin/dropin_caller: a C++ program built by hipcc that includes
-  only include/apsep/apsep.hpp and links libapsep.so.  It calls
+  only include/apsep/apsep.hpp and links libapsep.so (built twice: the
+  shims as shipped, and with the SPT names forced onto the cooperative
+  kernel, tests/dropin/dropin_caller_spt).  It calls
   launchWSTL / launchSPT (a device allocation per case, as the reference's
   src/main.cu:228-252 does), allocSPTScratch<int,128,4,INCL> + runSPT and
   allocWSTLScratch + runWSTL on one scratch for every case (as
@@ -23,7 +25,10 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-CALLER = os.path.join(HERE, "dropin", "dropin_caller")
+# the shims as shipped (the SPT names run the fastest solver), and built with
+# APSEP_SHIM_SPT_ALGO=APSEP_ALGO_SPT (the SPT names run the cooperative kernel)
+CALLERS = {"default": os.path.join(HERE, "dropin", "dropin_caller"),
+           "spt": os.path.join(HERE, "dropin", "dropin_caller_spt")}
 POISON = np.int32(np.uint32(0xABABABAB).view(np.int32))
 
 # entry bits of the caller (tests/dropin/dropin_caller.cpp)
@@ -39,10 +44,11 @@ def _log(*a):
     print("[dropin]", *a, file=sys.stderr, flush=True)
 
 
-def run_caller(tmp_path, cases, timeout=400):
-    """cases: list of (label, keys, mask).  Runs the C++ caller and checks
-    every answer it wrote against the oracle."""
-    assert os.path.exists(CALLER), "tests/dropin/dropin_caller is not built"
+def run_caller(tmp_path, cases, caller, timeout=400):
+    """cases: list of (label, keys, mask).  Runs the C++ caller (a key of
+    CALLERS) and checks every answer it wrote against the oracle."""
+    exe = CALLERS[caller]
+    assert os.path.exists(exe), "%s is not built" % exe
     cin = os.path.join(str(tmp_path), "cases.bin")
     cout = os.path.join(str(tmp_path), "answers.bin")
     with open(cin, "wb") as f:
@@ -50,8 +56,8 @@ def run_caller(tmp_path, cases, timeout=400):
         for _, keys, mask in cases:
             np.array([keys.size, mask], dtype=np.int32).tofile(f)
             np.ascontiguousarray(keys, dtype=np.int32).tofile(f)
-    _log("running the C++ caller on", len(cases), "cases")
-    r = subprocess.run([CALLER, cin, cout], capture_output=True, text=True, timeout=timeout)
+    _log("running the C++ caller (%s) on" % caller, len(cases), "cases")
+    r = subprocess.run([exe, cin, cout], capture_output=True, text=True, timeout=timeout)
     _log(r.stdout.strip().splitlines()[-1] if r.stdout.strip() else "(no output)")
     assert r.returncode == 0, "caller failed (%d):\n%s\n%s" % (r.returncode, r.stdout[-2000:],
                                                              r.stderr[-2000:])
@@ -87,7 +93,8 @@ def _known_cases():
         return json.load(f)["cases"]
 
 
-def test_caller_known_answers_and_stress(tmp_path):
+@pytest.mark.parametrize("caller", sorted(CALLERS))
+def test_caller_known_answers_and_stress(tmp_path, caller):
     """src/main.cu's correctness section through the shims: the structured
     cases on every entry point, then the mt19937 stress seeds -- 42 and 100
     through launchWSTL, 43 and 101 through launchSPT, as the reference
@@ -103,11 +110,12 @@ def test_caller_known_answers_and_stress(tmp_path):
                                      (101, 200, 131072, LAUNCH_SPT | RUN_SPT | RUN_SPT_LE)):
         for t, a in enumerate(oracle.stress_inputs(seed, trials, maxn)):
             cases.append(("seed=%d trial=%d" % (seed, t), a, mask))
-    n = run_caller(tmp_path, cases)
+    n = run_caller(tmp_path, cases, caller)
     _log("checked", n, "answer arrays")
 
 
-def test_caller_le_matrix(tmp_path):
+@pytest.mark.parametrize("caller", sorted(CALLERS))
+def test_caller_le_matrix(tmp_path, caller):
     """src/bench_spt_le.cu's matrix through the shims: allocSPTScratch at
     N = 2^25 for each predicate, runSPT<..,INCL> on 0xAB-poisoned output at
     n in {N, 1000003, 513, 512, 511, 1} (N itself for the random and the
@@ -129,7 +137,7 @@ def test_caller_le_matrix(tmp_path):
         for n in (N, 1000003, 513, 512, 511, 1):
             if n <= h.size:
                 cases.append(("%s n=%d" % (kind, n), h[:n], mask))
-    n = run_caller(tmp_path, cases, timeout=600)
+    n = run_caller(tmp_path, cases, caller, timeout=600)
     _log("checked", n, "answer arrays")
 
 
