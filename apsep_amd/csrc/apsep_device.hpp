@@ -54,6 +54,8 @@
 #include <stdint.h>
 #include <limits.h>
 
+#include <type_traits>
+
 namespace apsep {
 
 #ifndef APSEP_TILE_LANES
@@ -1129,52 +1131,73 @@ constexpr int LB_LEVELS = MAX_LEVELS;
 // spines (2 x RS int2)
 constexpr int FIX_LDS_INTS = WAVE + WAVE / 4 + 4 * RS;
 
-template <bool INCL>
-__device__ __forceinline__ int spine_answer_lds(const int2* sp, int cnt, int v) {
+// VAL: also return the answer's value in *val.
+template <bool INCL, bool VAL = false>
+__device__ __forceinline__ int spine_answer_lds(const int2* sp, int cnt, int v, int* val) {
     // lowest rank whose value qualifies (values decrease with rank): batches
     // of 16 ranks read at once and selected, highest rank first
     const int m = min(cnt, RS);
     const int4* p = reinterpret_cast<const int4*>(sp);
-    int pos = -1;
+    int pos = -1, pv = 0;
+    constexpr int B4 = VAL ? RS / 8 : RS / 4;  // dwordx4 reads per batch
 #pragma unroll
-    for (int h = 0; h < 2 && pos < 0 && h * (RS / 2) < m; h++) {
-        int4 w[RS / 4];
+    for (int h = 0; h < RS / 2 / B4 && pos < 0 && 2 * h * B4 < m; h++) {
+        int4 w[B4];
 #pragma unroll
-        for (int k = 0; k < RS / 4; k++) w[k] = p[h * (RS / 4) + k];
+        for (int k = 0; k < B4; k++) w[k] = p[h * B4 + k];
 #pragma unroll
-        for (int k = RS / 4 - 1; k >= 0; k--) {
-            const int r = h * (RS / 2) + 2 * k;
-            pos = (r + 1 < m && qual<INCL>(w[k].w, v)) ? w[k].z : pos;
-            pos = (r < m && qual<INCL>(w[k].y, v)) ? w[k].x : pos;
+        for (int k = B4 - 1; k >= 0; k--) {
+            const int r = 2 * (h * B4 + k);
+            const bool q1 = r + 1 < m && qual<INCL>(w[k].w, v);
+            pos = q1 ? w[k].z : pos;
+            if (VAL) pv = q1 ? w[k].w : pv;
+            const bool q0 = r < m && qual<INCL>(w[k].y, v);
+            pos = q0 ? w[k].x : pos;
+            if (VAL) pv = q0 ? w[k].y : pv;
         }
     }
+    if (VAL) *val = pv;
     return pos;
 }
 
-template <bool INCL, bool CO = false>
+// The nearest qualifying element of tile s for v (global index), read from
+// the tile's right spine (from LDS when sp_lds is given), past a truncated
+// spine from the chunk mins and the chunk.  VAL: also its value in *val.
+template <bool INCL, bool CO = false, bool VAL = false>
 __device__ __forceinline__ int answer_in_tile(const int* __restrict__ in, const TileOut& to,
-                                              int s, int v, const int2* sp_lds, int cnt) {
-    int pos = -1;
+                                              int s, int v, const int2* sp_lds, int cnt,
+                                              int* val = nullptr) {
+    int pos = -1, pv = 0;
     const int m = min(cnt, RS);
     if (sp_lds) {
-        pos = spine_answer_lds<INCL>(sp_lds, cnt, v);
+        pos = spine_answer_lds<INCL, VAL>(sp_lds, cnt, v, &pv);
     } else {
-        // two batches of 16 ranks (8 dwordx4 each), lowest rank wins
+        // batches of ranks read at once (two of 16 ranks, 8 dwordx4 each;
+        // with VAL, for callers that keep more registers live, four of 8),
+        // lowest rank wins
+        constexpr int B4 = VAL ? RS / 8 : RS / 4;  // dwordx4 loads per batch
         const int4* p = reinterpret_cast<const int4*>(to.spine + (size_t)s * RS);
 #pragma unroll
-        for (int h = 0; h < 2 && pos < 0; h++) {
-            int4 w[RS / 4];
+        for (int h = 0; h < RS / 2 / B4 && pos < 0; h++) {
+            int4 w[B4];
 #pragma unroll
-            for (int k = 0; k < RS / 4; k++) w[k] = ld_i4<CO>(&p[h * (RS / 4) + k]);
+            for (int k = 0; k < B4; k++) w[k] = ld_i4<CO>(&p[h * B4 + k]);
 #pragma unroll
-            for (int k = RS / 4 - 1; k >= 0; k--) {
-                const int r = h * (RS / 2) + 2 * k;
-                pos = (r + 1 < m && qual<INCL>(w[k].w, v)) ? w[k].z : pos;
-                pos = (r < m && qual<INCL>(w[k].y, v)) ? w[k].x : pos;
+            for (int k = B4 - 1; k >= 0; k--) {
+                const int r = 2 * (h * B4 + k);
+                const bool q1 = r + 1 < m && qual<INCL>(w[k].w, v);
+                pos = q1 ? w[k].z : pos;
+                if (VAL) pv = q1 ? w[k].w : pv;
+                const bool q0 = r < m && qual<INCL>(w[k].y, v);
+                pos = q0 ? w[k].x : pos;
+                if (VAL) pv = q0 ? w[k].y : pv;
             }
         }
     }
-    if (pos >= 0) return s * TILE + pos;
+    if (pos >= 0) {
+        if (VAL) *val = pv;
+        return s * TILE + pos;
+    }
     // deeper than the recorded spine: chunk mins, then the chunk
     int c = -1;
     if (CPT % 32 == 0) {
@@ -1195,8 +1218,10 @@ __device__ __forceinline__ int answer_in_tile(const int* __restrict__ in, const 
         }
     }
     if (c < 0) return -1;  // unreachable: the tile's min qualifies
-    const int e = last_in_32<INCL>(in, c * CHUNK + 32, v);
-    return e >= 0 ? e : last_in_32<INCL>(in, c * CHUNK, v);
+    int e = last_in_32<INCL>(in, c * CHUNK + 32, v);
+    if (e < 0) e = last_in_32<INCL>(in, c * CHUNK, v);
+    if (VAL && e >= 0) *val = in[e];
+    return e;
 }
 
 // This lane's share of the min over the tiles [a, b) (whole wave, uniform a
@@ -1277,6 +1302,226 @@ struct PagedMap {
         return reinterpret_cast<int*>(virt(pages[t].dst, t));
     }
 };
+
+// Per-wave state of a tile's fix-up that the search for an open element's
+// answer needs: the LDS window of the 64 tile mins before t (win), their
+// four-tile group mins (gm), the spines of t-1 and t-2 (sp) and their
+// counts, the min of tile t-1, and with two levels the tile group before the
+// window (mw2, per lane) and the level-1 values (k1, per lane) from gs on.
+struct FixCtx {
+    int t;
+    const int* win;
+    const int* gm;
+    const int2* sp;
+    int sc1, sc2;
+    int m_prev;
+    bool two;
+    int gs;
+    int mw2;
+    int k1;
+};
+
+// The answer (global index) of the open elements v of tile c.t held by the
+// lanes with `active`, all of which have one before the tile; VAL: also its
+// value in val.  Called by the whole wave.
+// LEAN: deep searches walk the levels from memory (find_tile_wave) instead
+// of using the preloaded mw2 / k1, which then need not stay live.
+template <bool INCL, bool CO, bool VAL, bool LEAN, class Map>
+__device__ __forceinline__ int resolve_open(const FixCtx& c, const Map& io, const Levels& L,
+                                            const TileOut& to, bool active, int v, int& val) {
+    const int lane = lane_id();
+    const int t = c.t;
+    // tile: the window first (tiles t-64 .. t-1) ...
+    int ts = -1;
+    if (active) {
+        int s = WAVE - 1;
+        if (!qual<INCL>(c.m_prev, v)) {
+            // the last qualifying group of four window tiles, then the last
+            // qualifying tile in it: two LDS rounds, whatever the distance
+            const int4* g4 = reinterpret_cast<const int4*>(c.gm);
+            const int4 ga = g4[0], gb = g4[1], gc = g4[2], gd = g4[3];
+            int g = -1;
+            sel4<INCL, 0>(g, ga.x, ga.y, ga.z, ga.w, v);
+            sel4<INCL, 4>(g, gb.x, gb.y, gb.z, gb.w, v);
+            sel4<INCL, 8>(g, gc.x, gc.y, gc.z, gc.w, v);
+            sel4<INCL, 12>(g, gd.x, gd.y, gd.z, gd.w, v);
+            s = -1;
+            if (g >= 0) {
+                const int4 w = reinterpret_cast<const int4*>(c.win)[g];
+                int k = -1;
+                sel4<INCL, 0>(k, w.x, w.y, w.z, w.w, v);
+                s = 4 * g + k;  // k >= 0: the group's min qualifies
+            }
+        }
+        // window entries of tiles < 0 hold INF, which never qualifies
+        // except under INCL with v == INF: exclude them explicitly
+        if (s >= 0 && t - WAVE + s >= 0) ts = t - WAVE + s;
+    }
+    // ... then, for the rare lanes whose tile is older, the whole wave walks
+    // the hierarchy for one of them at a time
+    u64 deep = __ballot(active && ts < 0);
+    while (deep) {
+        const int l = __ffsll(deep) - 1;
+        deep &= deep - 1;
+        const int vv = __shfl(v, l, WAVE);
+        int r = -1;
+        if (LEAN || !c.two) {
+            r = find_tile_wave<INCL, CO>(L, t - WAVE, vv);
+        } else if (t > WAVE) {
+            const u64 b0 = __ballot(c.gs + lane < t - WAVE && qual<INCL>(c.mw2, vv));
+            if (b0) {
+                r = c.gs + 63 - __clzll(b0);
+            } else {
+                const u64 b1 = __ballot(lane < (c.gs >> LOG_FAN) && qual<INCL>(c.k1, vv));
+                if (b1) {
+                    const int g = 63 - __clzll(b1);
+                    const int idx = (g << LOG_FAN) + lane;
+                    const u64 b2 = __ballot(idx < L.n[0] && qual<INCL>(ld_i<CO>(&L.mn0[idx]), vv));
+                    r = b2 ? (g << LOG_FAN) + 63 - __clzll(b2) : -1;
+                }
+            }
+        }
+        if (lane == l) ts = r;
+    }
+    int r = -1;
+    if (active && ts >= 0) {  // (ts < 0 is unreachable: P qualifies, so some tile does)
+        // spines of t-1 and t-2 are in LDS, older ones are read
+        const int2* spl = ts == t - 1 ? c.sp : ts == t - 2 ? c.sp + RS : nullptr;
+        const int cnt = ts == t - 1 ? c.sc1 : ts == t - 2 ? c.sc2 : ld_i<CO>(&to.scount[ts]);
+        r = answer_in_tile<INCL, CO, VAL>(io.src(ts), to, ts, v, spl, cnt, &val);
+    }
+    return r;
+}
+
+// Fix-up of a tile with more open elements than its recorded list holds.
+// P: the prefix-min before the tile (segmented: before it within the
+// segment running into it, which starts at s0); lim: tile-relative end of
+// that segment.
+template <bool INCL, bool SEG, bool CO, class Map>
+__device__ __forceinline__ void fix_rescan(const FixCtx& c, const Map& io, int n,
+                                           const Levels& L, const TileOut& to, int P, int s0,
+                                           int lim) {
+    // Rescan the tile, 4 keys per lane and 256 per round.  Open
+    // elements are the tile's prefix minima: an element is open iff no
+    // earlier element of the tile qualifies for it.  Their answers move
+    // monotonically: the answer c of an open element u also answers every
+    // later open element v with qual(key c, v), since every element between
+    // c and the tile failed u, which is at least as large as v, and no
+    // element of the tile before v qualifies for v.  So an element first
+    // tries the answer of the previous open element of its lane (for the
+    // lane's first one, of the last open element of the previous round, or
+    // in the first round of the tile's first open element), and only a miss
+    // is searched for.
+    const int lane = lane_id();
+    const int t = c.t;
+    const int tile_start = t * TILE;
+    const int* src = io.src(t) + tile_start;
+    int* dst = io.dst(t) + tile_start;
+    const int span = min(lim, n - tile_start);
+    int rm = INF;                       // min of the rounds before
+    int c_pos = -1, c_val = 0;          // carried answer (global index) and its value
+    bool c_ok = false;
+    for (int o = 0; o < span; o += 4 * WAVE) {
+        const int q = o + 4 * lane;  // tile-relative index of this lane's first key
+        int x[4];
+        if (q + 4 <= span) {
+            const int4 w = *reinterpret_cast<const int4*>(src + q);
+            x[0] = w.x; x[1] = w.y; x[2] = w.z; x[3] = w.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) x[k] = q + k < span ? src[q + k] : INF;
+        }
+        // prefix-min before each key: the rounds before, the earlier lanes
+        // (DPP scan: rows, then across rows), the lane's earlier keys.
+        // Every key of the range is real, so for any key but the tile's
+        // first the min is over real keys.
+        const int l1 = min(x[0], x[1]), l2 = min(l1, x[2]), l3 = min(l2, x[3]);
+        int inc = min(l3, dpp<DPP_ROW_SHR + 1>(INF, l3));
+        inc = min(inc, dpp<DPP_ROW_SHR + 2>(INF, inc));
+        inc = min(inc, dpp<DPP_ROW_SHR + 4>(INF, inc));
+        inc = min(inc, dpp<DPP_ROW_SHR + 8>(INF, inc));
+        inc = min(inc, dpp<DPP_ROW_BCAST15, 0xa>(INF, inc));
+        inc = min(inc, dpp<DPP_ROW_BCAST31, 0xc>(INF, inc));
+        const int rm_next = min(rm, lane63(inc));
+        const int ex = min(dpp<DPP_WAVE_SHR1>(INF, inc), rm);
+        unsigned need = 0;  // open keys with an answer before the tile
+        {
+            const int pm[4] = {ex, min(ex, x[0]), min(ex, l1), min(ex, l2)};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool open = q + k == 0 || !qual<INCL>(pm[k], x[k]);
+                if (q + k < span && open && qual<INCL>(P, x[k])) need |= 1u << k;
+            }
+        }
+        // Each lane answers its keys in order from a candidate while it
+        // qualifies: first the carried answer (usually for all four keys:
+        // one 16-byte store), after a miss the miss's own answer.  A miss is
+        // searched for; before any answer is carried (the tile's first
+        // round), only the round's first such key is, and its answer then
+        // serves every lane.  One search call site keeps the code small.
+        unsigned todo = need;
+        // the lane's leading keys that candidate (pos, val) qualifies for
+        auto take = [&](int pos, int val) {
+            const int a = SEG ? (pos >= s0 ? pos - s0 : -1) : pos;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (todo && __ffs(todo) - 1 == k && qual<INCL>(val, x[k])) {
+                    dst[q + k] = a;
+                    todo &= todo - 1;
+                }
+            }
+        };
+        if (c_ok) {
+            if (need == 0xFu && qual<INCL>(c_val, x[3])) {
+                // x[3] is the lane's smallest open key: all four qualify
+                const int a = SEG ? (c_pos >= s0 ? c_pos - s0 : -1) : c_pos;
+                *reinterpret_cast<int4*>(dst + q) = make_int4(a, a, a, a);
+                todo = 0;
+            } else {
+                take(c_pos, c_val);
+            }
+        }
+        int cp = c_pos, cv = c_val;
+        while (true) {
+            const u64 b = __ballot(todo != 0);
+            if (!b) break;
+            const int l0 = __ffsll(b) - 1;
+            const bool act = todo != 0 && (c_ok || lane == l0);
+            const int k = act ? __ffs(todo) - 1 : 0;
+            const int v = k == 0 ? x[0] : k == 1 ? x[1] : k == 2 ? x[2] : x[3];
+            int val = 0;
+            const int r = resolve_open<INCL, CO, true, true>(c, io, L, to, act, v, val);
+            if (act) {
+                dst[q + k] = SEG ? (r >= s0 ? r - s0 : -1) : r;
+                todo &= todo - 1;
+                cp = r;
+                cv = val;
+                if (r >= 0) take(r, val);
+            }
+            if (!c_ok) {
+                c_pos = readlane(r, l0);
+                c_val = readlane(val, l0);
+                c_ok = true;
+                if (c_pos < 0) break;  // unreachable: P qualifies for the key
+                if (lane != l0) {
+                    cp = c_pos;
+                    cv = c_val;
+                    take(c_pos, c_val);
+                }
+            }
+        }
+        // carry the answer of the round's last open key
+        const u64 b = __ballot(need != 0);
+        if (b) {
+            const int l = 63 - __clzll(b);
+            c_pos = readlane(cp, l);
+            c_val = readlane(cv, l);
+            c_ok = c_pos >= 0;
+        }
+        rm = rm_next;
+        if (!qual<INCL>(P, rm)) break;  // later open elements are <= rm
+    }
+}
 
 // Min of in[s0 .. t * TILE) for s0 < t * TILE (whole wave, uniform result):
 // the rest of s0's chunk, the rest of its tile's chunk mins, then whole
@@ -1386,103 +1631,18 @@ __device__ __forceinline__ void fix_tile(const Map& io, int n, int t, const Leve
     }
     if (lane < 2 * RS) sp[lane] = sw;
     wave_lds_sync();
+    const FixCtx fc{t, win, gm, sp, sc1, sc2, readlane(mw, WAVE - 1), two, gs, mw2, k1};
     if (cnt <= QS) {
         const bool active = lane < cnt && qual<INCL>(P, e.y) && (!SEG || e.x < lim);
-        const int v = e.y;
-        // tile: the window first (tiles t-64 .. t-1) ...
-        int ts = -1;
-        const int m_prev = readlane(mw, WAVE - 1);  // min of tile t - 1
+        int unused;
+        int r = resolve_open<INCL, CO, false, false>(fc, io, L, to, active, e.y, unused);
         if (active) {
-            int s = WAVE - 1;
-            if (!qual<INCL>(m_prev, v)) {
-                // the last qualifying group of four window tiles, then the
-                // last qualifying tile in it: two LDS rounds, whatever the
-                // distance
-                const int4* g4 = reinterpret_cast<const int4*>(gm);
-                const int4 ga = g4[0], gb = g4[1], gc = g4[2], gd = g4[3];
-                int g = -1;
-                sel4<INCL, 0>(g, ga.x, ga.y, ga.z, ga.w, v);
-                sel4<INCL, 4>(g, gb.x, gb.y, gb.z, gb.w, v);
-                sel4<INCL, 8>(g, gc.x, gc.y, gc.z, gc.w, v);
-                sel4<INCL, 12>(g, gd.x, gd.y, gd.z, gd.w, v);
-                s = -1;
-                if (g >= 0) {
-                    const int4 w = reinterpret_cast<const int4*>(win)[g];
-                    int k = -1;
-                    sel4<INCL, 0>(k, w.x, w.y, w.z, w.w, v);
-                    s = 4 * g + k;  // k >= 0: the group's min qualifies
-                }
-            }
-            // window entries of tiles < 0 hold INF, which never qualifies
-            // except under INCL with v == INF: exclude them explicitly
-            if (s >= 0 && t - WAVE + s >= 0) ts = t - WAVE + s;
-        }
-        // ... then, for the rare lanes whose tile is older, the whole wave
-        // walks the hierarchy for one of them at a time
-        u64 deep = __ballot(active && ts < 0);
-        while (deep) {
-            const int l = __ffsll(deep) - 1;
-            deep &= deep - 1;
-            const int vv = __shfl(v, l, WAVE);
-            int r = -1;
-            if (!two) {
-                r = find_tile_wave<INCL, CO>(L, t - WAVE, vv);
-            } else if (t > WAVE) {
-                const u64 b0 = __ballot(gs + lane < t - WAVE && qual<INCL>(mw2, vv));
-                if (b0) {
-                    r = gs + 63 - __clzll(b0);
-                } else {
-                    const u64 b1 = __ballot(lane < (gs >> LOG_FAN) && qual<INCL>(k1, vv));
-                    if (b1) {
-                        const int g = 63 - __clzll(b1);
-                        const int idx = (g << LOG_FAN) + lane;
-                        const u64 b2 = __ballot(idx < L.n[0] && qual<INCL>(ld_i<CO>(&L.mn0[idx]), vv));
-                        r = b2 ? (g << LOG_FAN) + 63 - __clzll(b2) : -1;
-                    }
-                }
-            }
-            if (lane == l) ts = r;
-        }
-        if (active) {
-            int r;
-            if (ts < 0) r = -1;  // unreachable: P qualifies, so some tile does
-            else if (ts == t - 1) r = answer_in_tile<INCL, CO>(io.src(ts), to, ts, v, sp, sc1);
-            else if (ts == t - 2) r = answer_in_tile<INCL, CO>(io.src(ts), to, ts, v, sp + RS, sc2);
-            else r = answer_in_tile<INCL, CO>(io.src(ts), to, ts, v, nullptr,
-                                              ld_i<CO>(&to.scount[ts]));
             if (SEG) r = r >= s0 ? r - s0 : -1;
             io.dst(t)[tile_start + e.x] = r;
         }
         return;
     }
-    // more open elements than recorded: rescan the tile.  Open elements are
-    // the tile's prefix minima: an element is open iff no earlier element of
-    // the tile qualifies for it (running prefix-min rm).
-    int rm = INF;
-    bool have = false;  // rm holds a real element (INCL: INF is a legal key)
-    // (offsets from the tile start: p0 + 64 would overflow near 2^31)
-    const int span = min(lim, n - tile_start);
-    for (int o = 0; o < span; o += WAVE) {
-        const int p0 = tile_start + o;
-        const bool valid = lane < span - o;
-        const int idx = valid ? p0 + lane : p0;
-        const int xv = valid ? io.src(t)[idx] : INF;
-        const int inc = wave_incl_min(xv);
-        int ex = wave_shfl_up(inc, 1);
-        const bool ex_real = lane > 0 || have;
-        ex = (lane == 0) ? rm : min(ex, rm);
-        const bool open = valid && !(ex_real && qual<INCL>(ex, xv));
-        if (open && qual<INCL>(P, xv)) {
-            const int ts = find_tile<INCL, CO>(L, t, xv);
-            int r = (ts < 0) ? -1 : answer_in_tile<INCL, CO>(io.src(ts), to, ts, xv, nullptr,
-                                                            ld_i<CO>(&to.scount[ts]));
-            if (SEG) r = r >= s0 ? r - s0 : -1;
-            io.dst(t)[idx] = r;
-        }
-        rm = min(rm, __shfl(inc, WAVE - 1, WAVE));
-        have = true;
-        if (!qual<INCL>(P, rm)) break;  // later open elements are <= rm
-    }
+    fix_rescan<INCL, SEG, CO>(fc, io, n, L, to, P, s0, lim);
 }
 
 // ---------------------------------------------------------------------------

@@ -154,6 +154,15 @@ def test_tile_and_level_boundaries(torch_mod, algo):
             check(solve_on(s, torch, a, True), a, True, "%s n=%d intmax incl" % (algo, n))
 
 
+def _pairs_distinct(n, tile=8192):
+    i = np.arange(n, dtype=np.int64)
+    j = i % tile
+    pair = i // (2 * tile)
+    base = 3 * (n - pair * 2 * tile)  # later pairs lower: every pair starts a new minimum
+    second = (i // tile) % 2 == 1
+    return np.where(second, base + 2 * (tile - 1 - j) + 1, base + 2 * j).astype(np.int32)
+
+
 @pytest.mark.parametrize("algo", ALGOS)
 def test_adversarial_deep_and_overflow(torch_mod, algo):
     """Inputs whose open elements need long searches or exceed the recorded
@@ -171,6 +180,13 @@ def test_adversarial_deep_and_overflow(torch_mod, algo):
         "sawtooth-desc": ((n - i) % 3000 + (i % 2)).astype(np.int32),
         "min-plateau": np.where(i % 2048 < 100, 1, 5 + (i % 7)).astype(np.int32),
         "two-level-descent": (((n - i) // 2048) * 4096 + (i % 2048)).astype(np.int32),
+        # tile pairs: an ascending tile of even values, then a descending
+        # tile of odd values: every open element of the second tile has its
+        # own answer in the first, past its recorded spine
+        "pairs-distinct-answers": _pairs_distinct(n),
+        # descending runs of two equal keys after a small first key: "<"
+        # leaves both keys of a pair open, "<=" answers the second in-tile
+        "zero-then-paired-descent": np.where(i == 0, 0, (n - i) // 2 + 5).astype(np.int32),
     }
     with A.Scratch(algo, n) as s:
         for name, a in cases.items():

@@ -86,6 +86,40 @@ def test_edge_lengths(torch_mod, kind, incl, lanes):
 
 
 @pytest.mark.parametrize("lanes", [128, 256, 512])
+@pytest.mark.parametrize("incl", [False, True])
+def test_segments_with_open_runs(torch_mod, incl, lanes):
+    """Segments spanning several tiles whose later tiles are open throughout
+    (more open elements than a tile records): a small first key then a
+    descent (every answer is the segment's first key), block-descending
+    runs, and descending segments after a segment of small keys (no answer
+    may cross into the earlier segment)."""
+    import apsep_amd as A
+    rng = np.random.default_rng(17)
+    parts, lengths = [], []
+    for j in range(24):
+        L = int(rng.integers(3 * 8192, 6 * 8192))
+        i = np.arange(L, dtype=np.int64)
+        kind = j % 4
+        if kind == 0:
+            seg = np.where(i == 0, 3, 10 * L - i)
+        elif kind == 1:
+            seg = oracle.fill("blockdesc", L, seed=40 + j).astype(np.int64)
+        elif kind == 2:
+            seg = np.full(L, 1, dtype=np.int64)  # small keys ...
+        else:
+            seg = 5 * L - i  # ... then a descent: every answer is -1
+        parts.append(seg.astype(np.int32))
+        lengths.append(L)
+    a = np.concatenate(parts)
+    off = _offsets(lengths)
+    with A.BatchPlan(a.shape[0], len(lengths), incl) as p:
+        p.set_tile_lanes(lanes)
+        _log("open runs", incl, lanes, a.shape[0])
+        _check(_solve(p, torch_mod, a, off), a, off, incl,
+               "open runs incl=%d lanes=%d" % (incl, lanes))
+
+
+@pytest.mark.parametrize("lanes", [128, 256, 512])
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_many_small_segments(torch_mod, seed, lanes):
     """Thousands of segments shorter than a lane / a tile (many starts per
