@@ -249,9 +249,18 @@ __global__ __launch_bounds__(NT) void k_lookback(const int* __restrict__ in,
         if (active) out[tile_start + e.x] = ts < 0 ? -1 : lb_answer_in_tile<INCL>(V, in, ts, val);
         return;
     }
-    // more open elements than recorded: rescan the tile (prefix minima)
+    // More open elements than recorded: rescan the tile (its prefix minima
+    // are the open elements).  Their answers move monotonically: the answer
+    // c of an open element also answers every later open element v with
+    // qual(key c, v) (everything between c and the tile failed the earlier,
+    // larger element; nothing in the tile before v qualifies for v).  So the
+    // answer of the last open element seen is carried, keys it qualifies
+    // for take it, and only the others are searched for, one at a time
+    // from the first, each new answer serving the keys after it.
     int rm = INF;
     bool have = false;
+    int c_pos = -1, c_val = 0;  // carried answer and its key
+    bool c_ok = false;
     // (offsets from the tile start: p0 + 64 would overflow near 2^31)
     const int span = min(TILE, n - tile_start);
     for (int o = 0; o < span; o += WAVE) {
@@ -259,20 +268,45 @@ __global__ __launch_bounds__(NT) void k_lookback(const int* __restrict__ in,
         const bool valid = lane < span - o;
         const int idx = valid ? p0 + lane : p0;
         const int xv = valid ? in[idx] : INF;
-        const int inc = wave_incl_min(xv);
-        int ex = wave_shfl_up(inc, 1);
+        int inc = min(xv, dpp<DPP_ROW_SHR + 1>(INF, xv));
+        inc = min(inc, dpp<DPP_ROW_SHR + 2>(INF, inc));
+        inc = min(inc, dpp<DPP_ROW_SHR + 4>(INF, inc));
+        inc = min(inc, dpp<DPP_ROW_SHR + 8>(INF, inc));
+        inc = min(inc, dpp<DPP_ROW_BCAST15, 0xa>(INF, inc));
+        inc = min(inc, dpp<DPP_ROW_BCAST31, 0xc>(INF, inc));
         const bool ex_real = lane > 0 || have;
-        ex = (lane == 0) ? rm : min(ex, rm);
+        const int ex = min(dpp<DPP_WAVE_SHR1>(INF, inc), rm);
         const bool open = valid && !(ex_real && qual<INCL>(ex, xv));
-        const bool act = open && qual<INCL>(P, xv);
-        u64 todo = __ballot(act);
-        while (todo) {
-            const int l = __ffsll(todo) - 1;
-            todo &= todo - 1;
-            const int r = lb_find_tile<INCL>(V, t, __shfl(xv, l, WAVE));
-            if (lane == l) out[idx] = r < 0 ? -1 : lb_answer_in_tile<INCL>(V, in, r, xv);
+        bool todo = open && qual<INCL>(P, xv);
+        if (todo && c_ok && qual<INCL>(c_val, xv)) {
+            out[idx] = c_pos;
+            todo = false;
         }
-        rm = min(rm, __shfl(inc, WAVE - 1, WAVE));
+        u64 m = __ballot(todo);
+        while (m) {
+            const int l = __ffsll(m) - 1;
+            const int v = __shfl(xv, l, WAVE);
+            const int r = lb_find_tile<INCL>(V, t, v);
+            int a = -1, av = 0;
+            if (lane == l && r >= 0) {
+                a = lb_answer_in_tile<INCL>(V, in, r, v);
+                if (a >= 0) av = in[a];
+            }
+            c_pos = readlane(a, l);
+            c_val = readlane(av, l);
+            c_ok = c_pos >= 0;
+            if (lane == l) {
+                out[idx] = c_pos;
+                todo = false;
+            }
+            // the keys after lane l that this answer qualifies for
+            if (todo && lane > l && c_ok && qual<INCL>(c_val, xv)) {
+                out[idx] = c_pos;
+                todo = false;
+            }
+            m = __ballot(todo);
+        }
+        rm = min(rm, lane63(inc));
         have = true;
         if (!qual<INCL>(P, rm)) break;  // later open elements are <= rm
     }
