@@ -26,7 +26,12 @@ Printed JSON line (rank 0, one line):
                re-run on one array (its data partly cache-resident)
   cpu_baseline the oracle's monotone stack (1 thread) on a bounded sample of
                the same inputs, timed on this host
-  per_class_*  each class alone: solves back to back between one event pair
+  per_class_*  each class alone: solves back to back between one event pair,
+               over three input/output pairs of the class in turn (_warm: on
+               one pair)
+  structured   the tie- and structure-heavy arrays at N = 2^28 (uniform
+               [0,16), sawtooth i mod 4096, block-descending runs of 2^16):
+               useful GB/s per array, the solve's kernels, oracle parity
   batch        the batched configuration (8192 segments, log-uniform lengths
                64..2^20, ~8.7e8 keys): arrays/s and useful GB/s of one
                segmented solve, against one apsep_run and one apsep_launch
@@ -560,6 +565,89 @@ def bench_tune(args, torch, apsep_amd):
         "config": {"workload": "autotuner size classes 2^14..2^28"}, "rows": rows}), flush=True)
 
 
+def structured_inputs(torch, n: int, seed: int = 11):
+    """The tie- and structure-heavy single arrays, built on the device:
+    uniform keys in [0, 16), a sawtooth i mod 4096, and block-descending
+    runs of 2^16 keys (a[i] = off[r] + 2^16 - 1 - i mod 2^16, off[r] uniform
+    in [0, 2^31 - 2^16))."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    i = torch.arange(n, dtype=torch.int64, device="cuda")
+    off = torch.randint(0, 2**31 - 2**16, ((n + 65535) >> 16,), generator=g,
+                        dtype=torch.int64, device="cuda")
+    xs = {
+        "dup16": torch.randint(0, 16, (n,), generator=g, dtype=torch.int32, device="cuda"),
+        "sawtooth": (i % 4096).to(torch.int32),
+        "blockdesc": (off[i >> 16] + (65535 - (i & 65535))).to(torch.int32),
+    }
+    del i, off
+    return xs
+
+
+def run_structured(args, torch, apsep_amd, cpu: bool):
+    """Solve time of each structured input at N = 2^28 (`structured_reps`
+    solves back to back between one event pair, after two untimed ones),
+    the split into the solve's kernels, and with `cpu` a check of every
+    answer against the oracle, whose time on the same arrays is the CPU
+    baseline."""
+    n = args.structured_n
+    stream = torch.cuda.current_stream()
+    xs = structured_inputs(torch, n)
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    rows = {}
+    ok, secs = True, 0.0
+    with apsep_amd.Scratch(args.algo, n) as s:
+        for k, x in xs.items():
+            for _ in range(2):
+                s.run(x, out, stream=stream)
+            ms = _timed(torch, stream, lambda: s.run(x, out, stream=stream),
+                        args.structured_reps)
+            t = s.time_kernels(x, out, args.structured_reps, stream=stream)
+            s.check()
+            rows[k] = {"us": round(ms * 1e3, 1),
+                       "useful_GBps": round(8.0 * n / (ms * 1e-3) / 1e9, 1),
+                       "kernel_us": _kernel_us(t)}
+            print("structured %s: %s" % (k, json.dumps(rows[k])), file=sys.stderr, flush=True)
+            if cpu:
+                import numpy as np
+                import oracle
+                host = x.cpu().numpy()
+                t0 = time.perf_counter()
+                exp = oracle.apsep(host)
+                secs += time.perf_counter() - t0
+                ok = ok and bool(np.array_equal(exp, out.cpu().numpy()))
+                del host, exp
+    rec = {"workload": "tie- and structure-heavy single arrays, N=2^%d int32: uniform [0,16), "
+                       "sawtooth i mod 4096, block-descending runs of 2^16"
+                       % (n.bit_length() - 1),
+           "n": n, "per_kind": rows,
+           "useful_GBps": round(3 * 8.0 * n / (sum(r["us"] for r in rows.values()) * 1e-6)
+                                / 1e9, 1)}
+    if cpu:
+        rec["parity_checked"] = ok
+        rec["cpu_baseline"] = {"value": round(3 * 8.0 * n / secs / 1e9, 3), "unit": "GB/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "each of the 3 arrays once (%.1f s of 1-thread "
+                                         "monotone stack, oracle/apsep_oracle.c -O3)" % secs}
+        if not ok:
+            print("bench.py: structured answers differ from the oracle", file=sys.stderr)
+    del xs, out
+    torch.cuda.empty_cache()
+    return rec
+
+
+def bench_structured(args, torch, apsep_amd):
+    rec = run_structured(args, torch, apsep_amd, not args.no_cpu)
+    print(json.dumps({
+        "metric": "useful GB/s, structure-heavy single arrays at N=2^28 int32",
+        "value": rec["useful_GBps"], "unit": "GB/s", "n_gpus": 1,
+        "steps": args.structured_reps, "warmup": 2, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (device-generated, torch seed 11)",
+        "config": {"workload": rec["workload"], "n": rec["n"], "algo": args.algo},
+        **{k: v for k, v in rec.items() if k not in ("workload", "n")}}), flush=True)
+
+
 def bench_sweep(args, torch, apsep_amd):
     """Useful GB/s of the plain solve for N = 2^14 .. 2^27 on the three
     classes, cache-warm (repeated solves of the same array) and with the
@@ -747,11 +835,14 @@ def main():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="rocprofv3 FETCH/WRITE summary (bytes per launch)")
     ap.add_argument("--workload", default="single",
-                    choices=["single", "batch", "serve", "sweep", "tune"],
-                    help="single: the plain N=2^25 solve (headline) with the batch and "
-                         "serve sections; batch: the segmented CSR batch alone; serve: "
-                         "the serving engine alone; sweep: sizes 2^14..2^27, cache-warm "
-                         "and flushed; tune: the autotuner's choice per size class")
+                    choices=["single", "batch", "serve", "sweep", "tune", "structured"],
+                    help="single: the plain N=2^25 solve (headline) with the structured, "
+                         "batch and serve sections; batch: the segmented CSR batch alone; "
+                         "serve: the serving engine alone; sweep: sizes 2^14..2^27, "
+                         "cache-warm and flushed; tune: the autotuner's choice per size "
+                         "class; structured: the structure-heavy arrays at N=2^28 alone")
+    ap.add_argument("--structured-n", type=int, default=1 << 28)
+    ap.add_argument("--structured-reps", type=int, default=10)
     ap.add_argument("--nseg", type=int, default=8192)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-warmup", type=int, default=3)
@@ -791,10 +882,15 @@ def main():
     if args.workload == "tune":
         bench_tune(args, torch, apsep_amd)
         return
+    if args.workload == "structured":
+        bench_structured(args, torch, apsep_amd)
+        return
 
     rec = run_single(args, torch, apsep_amd)
     print("bench: single done (%.1f GB/s)" % rec["value"], file=sys.stderr, flush=True)
     if not args.no_extras:
+        rec["structured"] = run_structured(args, torch, apsep_amd, not args.no_cpu)
+        print("bench: structured done", file=sys.stderr, flush=True)
         rec["batch"] = run_batch(args, torch, apsep_amd, not args.no_cpu)
         print("bench: batch done", file=sys.stderr, flush=True)
         rec["serve"] = run_serve(args, torch, apsep_amd, not args.no_cpu)
