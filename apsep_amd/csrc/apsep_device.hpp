@@ -217,6 +217,7 @@ struct TileOut {
 // per CU).
 struct TileSmem {
     int4 val4[K4][NT];          // the tile's values
+    int4 gmin[NT];              // each lane's four group-of-4 mins
     int win[4][NT];             // lane-min windows inside rows
     int4 rmin[NWAVE];           // row mins of each wave
     int wmin[NWAVE];            // wave mins
@@ -409,16 +410,19 @@ __device__ int find_tile(const Levels& L, int t, int v) {
 template <bool INCL, int J0>
 __device__ __forceinline__ void sel4(int& r, int a0, int a1, int a2, int a3, int x);
 
+// Two steps instead of sixteen selects: the last group of four whose min
+// qualifies (the lane's group mins, one read), then the last qualifying
+// value in that group (one more read).
 template <bool INCL>
 __device__ __forceinline__ int last_in_lane(const TileSmem& sm, int lt, int v) {
-    int r = -1;
-    const int4 w0 = sm.val4[0][lt], w1 = sm.val4[1][lt];
-    const int4 w2 = sm.val4[2][lt], w3 = sm.val4[3][lt];
-    sel4<INCL, 0>(r, w0.x, w0.y, w0.z, w0.w, v);
-    sel4<INCL, 4>(r, w1.x, w1.y, w1.z, w1.w, v);
-    sel4<INCL, 8>(r, w2.x, w2.y, w2.z, w2.w, v);
-    sel4<INCL, 12>(r, w3.x, w3.y, w3.z, w3.w, v);
-    return r;
+    const int4 g = sm.gmin[lt];
+    int q = -1;
+    sel4<INCL, 0>(q, g.x, g.y, g.z, g.w, v);
+    // the read is unconditional (group 0 when none qualifies)
+    const int4 w = sm.val4[max(q, 0)][lt];
+    int j = -1;
+    sel4<INCL, 0>(j, w.x, w.y, w.z, w.w, v);
+    return q < 0 ? -1 : 4 * q + j;  // j >= 0: the group's min qualifies
 }
 
 // Answer (tile-relative element index) of the query value `val` asked by
@@ -627,13 +631,17 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         }
         res[i] = base + r;
     }
-    int lmin = v[0];
+    int gm[K4];
 #pragma unroll
-    for (int i = 1; i < K; i++) lmin = min(lmin, v[i]);
+    for (int q = 0; q < K4; q++)
+        gm[q] = min(min(v[4 * q], v[4 * q + 1]), min(v[4 * q + 2], v[4 * q + 3]));
+    const int lmin = min(min(gm[0], gm[1]), min(gm[2], gm[3]));
+    static_assert(K4 == 4, "group mins: one int4 per lane");
 
 #pragma unroll
     for (int q = 0; q < K4; q++)
         sm.val4[q][tid] = make_int4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    sm.gmin[tid] = make_int4(gm[0], gm[1], gm[2], gm[3]);
 
     // 2. lane-min windows inside rows, wave prefix / suffix mins (DPP)
     const int w2 = min(lmin, dpp<DPP_ROW_SHR + 1>(INF, lmin));

@@ -847,9 +847,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-warmup", type=int, default=3)
     ap.add_argument("--jobs", type=int, default=None,
-                    help="serve: jobs per rate (default 20000 alone, 4000 in the headline)")
+                    help="serve: jobs per rate (default 20000 alone, 10000 in the headline)")
     ap.add_argument("--payloads", type=int, default=None,
-                    help="serve: distinct jobs (default 1000 alone, 400 in the headline)")
+                    help="serve: distinct jobs (default 1000)")
     ap.add_argument("--rates", default="10000,50000,150000,400000",
                     help="serve: offered jobs/s")
     args = ap.parse_args()
@@ -858,9 +858,9 @@ def main():
     if args.warmup is None:
         args.warmup = 50 if args.workload == "single" else 3
     if args.jobs is None:
-        args.jobs = 20000 if args.workload == "serve" else 4000
+        args.jobs = 20000 if args.workload == "serve" else 10000
     if args.payloads is None:
-        args.payloads = 1000 if args.workload == "serve" else 400
+        args.payloads = 1000
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not "
               "supported (the solver does not shard across GPUs)" % args.gpus,
