@@ -3,6 +3,8 @@
 // Argument checks run before any HIP call, so every error path is testable
 // on a machine without a GPU.  Nothing here prints or exits.
 #include <hip/hip_runtime.h>
+
+#include <atomic>
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
@@ -24,6 +26,15 @@ void record_hip_error(hipError_t e, const char* func, int line) {
              hipGetErrorString(e), func, line);
 }
 
+static std::atomic<long long> g_coop_refusals{0};
+
+void note_cooperative_refusal(hipError_t e) {
+    g_coop_refusals.fetch_add(1, std::memory_order_relaxed);
+    snprintf(g_last_error, sizeof g_last_error,
+             "%s (%s) from hipLaunchCooperativeKernel: solved by the multi-launch kernels",
+             hipGetErrorName(e), hipGetErrorString(e));
+}
+
 static void record_device_error(int code) {
     snprintf(g_last_error, sizeof g_last_error,
              "device-side error word %d: a grid barrier or look-back wait gave up", code);
@@ -34,6 +45,8 @@ static void record_device_error(int code) {
 extern "C" {
 
 const char* apsep_last_error(void) { return g_last_error[0] ? g_last_error : "none"; }
+
+int64_t apsep_cooperative_refusals(void) { return g_coop_refusals.load(std::memory_order_relaxed); }
 
 const char* apsep_status_string(apsep_status s) {
     switch (s) {

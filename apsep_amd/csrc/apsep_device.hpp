@@ -74,16 +74,9 @@ constexpr int CPT = TILE / CHUNK;  // chunk mins per tile (128 at 512 lanes)
 constexpr int QS = 32;             // recorded open elements per tile
 constexpr int RS = 32;             // recorded right-spine entries per tile
 constexpr int QCAP = 192;          // per-wave LDS queue of cross-lane queries
-constexpr int XQ = 3;              // further queries per lane tried on the previous lane
-// segmented solves try them only with 256-lane tiles (run at 5 waves per
-// SIMD, room to spare); 512-lane segmented tiles run at 6 and would spill
-constexpr int XQ_SEG = NT <= 256 ? XQ : 0;
-#ifndef APSEP_XQ_SPT
-#define APSEP_XQ_SPT 2
-#endif
-// the persistent launch's tile loop holds a few more registers live: with
-// XQ_SPT checks its tile phase fits 80 VGPRs without spilling to memory
-constexpr int XQ_SPT = APSEP_XQ_SPT;
+// further queries per lane tried on the previous lane (measured on 2^25
+// random keys: 3 beats 2 and 4), in every solver and tile shape
+constexpr int XQ = 3;
 constexpr int FAN = 64;            // hierarchy fan-out
 constexpr int LOG_FAN = 6;
 constexpr int MAX_LEVELS = 5;      // 64^5 tiles >> 2^31 / TILE
@@ -769,11 +762,10 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     // round's cost each.  The low 16 bits of xa mark the elements so
     // answered; the answer of the k-th of them is at offset
     // (xa >> (16 + 4k)) & 15 of the previous lane.  They are written with
-    // the staged or patched outputs below.  (Segmented solves: XQ_SEG;
-    // the persistent launch: XQ_SPT.)
+    // the staged or patched outputs below.
     unsigned xa = 0;
 #pragma unroll
-    for (int k = 0; k < (SEG ? XQ_SEG : CO ? XQ_SPT : XQ); k++) {
+    for (int k = 0; k < XQ; k++) {
         if (near && qmask) {
             const int i = __builtin_ctz(qmask);
             const int r = last_in_lane<INCL>(sm, tid - 1, tile_value(sm, tid * K + i));
@@ -835,7 +827,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         }
         unsigned xm = xa & 0xFFFFu;
 #pragma unroll
-        for (int k = 0; k < (SEG ? XQ_SEG : CO ? XQ_SPT : XQ); k++)
+        for (int k = 0; k < XQ; k++)
             if (xm) {
                 const int i = __builtin_ctz(xm);
                 xm &= xm - 1;
@@ -937,7 +929,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         }
         unsigned xm = xa & 0xFFFFu;
 #pragma unroll
-        for (int x = 0; x < (SEG ? XQ_SEG : CO ? XQ_SPT : XQ); x++)
+        for (int x = 0; x < XQ; x++)
             if (xm) {
                 const int i = __builtin_ctz(xm), q = i >> 2;
                 xm &= xm - 1;

@@ -365,6 +365,15 @@ static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
     void* args[] = {(void*)&in, (void*)&out, (void*)&n, (void*)&to, (void*)&L, (void*)&bar};
     hipError_t e = hipLaunchCooperativeKernel((const void*)k_spt<INCL>, dim3(grid), dim3(NT),
                                               args, 0, st);
+    if (e == hipErrorIllegalState) {
+        // The runtime refused the cooperative launch and nothing ran (seen
+        // now and then in a process making many cooperative launches while
+        // another process on the device also made some): the multi-launch
+        // kernels solve on the same scratch instead, with the same answers.
+        (void)hipGetLastError();
+        note_cooperative_refusal(e);
+        return run_wstl<INCL>(s, in, out, n, st);
+    }
     if (e != hipSuccess) {
         APSEP_CLEAR_HIP_ERROR();
         return APSEP_ERR_LAUNCH;

@@ -66,6 +66,12 @@ const char* apsep_version(void);
  * the next failure. */
 const char* apsep_last_error(void);
 
+/* No counterpart in the reference; diagnostics.  The number of SPT solves
+ * in this process whose cooperative launch the HIP runtime refused
+ * (hipErrorIllegalState: nothing ran) and that the multi-launch kernels
+ * solved instead, on the same scratch and stream, with the same answers. */
+int64_t apsep_cooperative_refusals(void);
+
 /* Replaces allocWSTLScratch (src/apsep.cuh:2520-2537) and allocSPTScratch
  * (src/apsep.cuh:3013-3040): device scratch sized for up to n_cap elements,
  * bound to the current device.  n_cap may be 0 (then only n = 0 runs). */
