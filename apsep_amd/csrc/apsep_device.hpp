@@ -251,12 +251,13 @@ struct SegLane {
 
 // Answer `res` (global index or -1) of the lane's element i, rebased to its
 // segment (identity for plain solves).
+// (res - st is below -1 for every answer before st, -1 included: st >= 0)
 template <bool SEG>
 __device__ __forceinline__ int seg_rel(int res, int i, int base, const SegLane& sl) {
     if (!SEG) return res;
     const unsigned m = sl.wallmask & ((2u << i) - 1u);
     const int st = m ? base + 31 - __builtin_clz(m) : sl.incoming;
-    return res >= st ? res - st : -1;
+    return max(res - st, -1);
 }
 
 __device__ __forceinline__ int tile_value(const TileSmem& sm, int e) {
@@ -910,13 +911,25 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     // 56 -> 24 on descending input, measured, against an XOR by q << 2.)
     if (staged) {
         int* sv = reinterpret_cast<int*>(&sm.val4[0][0]);
+        if (!SEG || __ballot(sl.wallmask != 0) == 0) {
+            // no segment starts inside this wave's elements (nearly every
+            // wave of a batch): every answer is rebased to the same start
+            const int st = SEG ? sl.incoming : 0;
 #pragma unroll
-        for (int q = 0; q < K4; q++)
-            sm.val4[q][wave * WAVE + (lane ^ (q << 1))] =
-                make_int4(seg_rel<SEG>(res[4 * q], 4 * q, base, sl),
-                          seg_rel<SEG>(res[4 * q + 1], 4 * q + 1, base, sl),
-                          seg_rel<SEG>(res[4 * q + 2], 4 * q + 2, base, sl),
-                          seg_rel<SEG>(res[4 * q + 3], 4 * q + 3, base, sl));
+            for (int q = 0; q < K4; q++)
+                sm.val4[q][wave * WAVE + (lane ^ (q << 1))] =
+                    SEG ? make_int4(max(res[4 * q] - st, -1), max(res[4 * q + 1] - st, -1),
+                                    max(res[4 * q + 2] - st, -1), max(res[4 * q + 3] - st, -1))
+                        : make_int4(res[4 * q], res[4 * q + 1], res[4 * q + 2], res[4 * q + 3]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < K4; q++)
+                sm.val4[q][wave * WAVE + (lane ^ (q << 1))] =
+                    make_int4(seg_rel<SEG>(res[4 * q], 4 * q, base, sl),
+                              seg_rel<SEG>(res[4 * q + 1], 4 * q + 1, base, sl),
+                              seg_rel<SEG>(res[4 * q + 2], 4 * q + 2, base, sl),
+                              seg_rel<SEG>(res[4 * q + 3], 4 * q + 3, base, sl));
+        }
         unsigned m = qmask;
         int k = qoff;
         while (m) {
