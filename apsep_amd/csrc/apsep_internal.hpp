@@ -69,9 +69,10 @@ struct apsep_batch_plan {
 namespace apsep {
 
 // Tile lanes for a batch of `total` keys when neither the caller nor the
-// tuner chose (measured on MI355X: 128-lane tiles are fastest up to 2^26
-// keys, where a batch does not fill the GPU with 256-lane tiles; 256 beyond).
-inline int default_batch_lanes(int64_t total) { return total <= (1ll << 26) ? 128 : 256; }
+// tuner chose (measured on MI355X: 128-lane tiles are fastest up to about
+// 2^24 keys, where a batch does not fill the GPU with 256-lane tiles; 256
+// beyond).
+inline int default_batch_lanes(int64_t total) { return total <= (1ll << 24) ? 128 : 256; }
 
 // The HIP error behind the calling thread's last failing status: recorded
 // where the library clears HIP's sticky error (apsep_last_error reports it).

@@ -212,7 +212,7 @@ apsep_status apsep_batch_plan_use_graph(apsep_batch_plan* p, int enable);
  * src/apsep.cuh:2548, 3052); set_tile_lanes drops a recorded graph, and it
  * and apsep_batch_plan_use_graph switch the tuned table (below) off.  Until
  * the caller sets a shape, apsep_run_segmented picks it per batch by size
- * (128 lanes up to 2^26 keys, 256 beyond: the faster shapes measured on
+ * (128 lanes up to 2^24 keys, 256 beyond: the faster shapes measured on
  * MI355X); apsep_run_jobs always runs the current shape, whose tile size
  * the caller's job table is built for. */
 apsep_status apsep_batch_plan_set_tile_lanes(apsep_batch_plan* p, int32_t lanes);
