@@ -1616,6 +1616,9 @@ __device__ __forceinline__ void fix_tile(const Map& io, int n, int t, const Leve
                         (long long)tile_start + TILE) - tile_start);
         if (s0 >= tile_start) return;
     }
+    // no open element (every tile of an ascending run): nothing to fix, and
+    // a segmented tile skips its prefix-min's round of loads
+    if (cnt == 0) return;
     // exclusive prefix-min of tile t: level 0 from the window, upper levels
     // from the keys
     int p = (tw >= ((t >> LOG_FAN) << LOG_FAN)) ? mw : INF;
