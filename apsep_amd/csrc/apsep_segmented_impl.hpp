@@ -44,8 +44,13 @@ constexpr unsigned SEG_EPOCH = 1u;
 constexpr int SEG_FIX_WG = 256;  // four tiles per group, up to 8 waves per SIMD (as k_fix_pass)
 
 // tile -> segment holding the tile's first element, and that segment's
-// start and end (one wave per segment, its lanes over the segment's tiles:
-// a segment of 2^20 keys covers 256 tiles); level-1 keys emptied
+// start and end; level-1 keys emptied.  A lane per segment: a wave reads 64
+// segments' offsets in one coalesced load, each lane writes the map entries
+// of its segment when it holds at most two tile starts (every short
+// segment), and the wave writes those of the longer ones together, one
+// segment at a time, its lanes over the segment's tiles (a segment of 2^20
+// keys covers 256 tiles).  Every tile start lies in exactly one segment, so
+// every map entry is written once.
 __global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__ offs,
                                                    int nseg, int* __restrict__ tmap,
                                                    int2* __restrict__ tbound, int nt,
@@ -55,16 +60,36 @@ __global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__
     for (int i = gid; i < nkey1; i += stride) key1[key_slot(1, i)] = ~0ull;
     const int lane = (int)threadIdx.x & (WAVE - 1);
     const long long cap = (long long)nt * TILE;
-    for (int k = gid >> 6; k < nseg; k += stride >> 6) {  // uniform per wave
-        const long long s = max(offs[k], 0ll);
-        const long long e = min(offs[k + 1], cap);
-        if (s >= e) continue;
-        const int t0 = (int)((s + TILE - 1) / TILE);
-        const int t1 = (int)((e - 1) / TILE);
+    for (int k0 = (gid >> 6) * WAVE; k0 < nseg; k0 += (stride >> 6) * WAVE) {  // uniform per wave
+        const int k = k0 + lane;
+        long long s = 0, e = 0;
+        if (k < nseg) {
+            s = max(offs[k], 0ll);
+            e = min(offs[k + 1], cap);
+        }
+        int t0 = 0, t1 = -1;  // the tiles whose first element lies in [s, e)
+        if (s < e) {
+            t0 = (int)((s + TILE - 1) / TILE);
+            t1 = (int)((e - 1) / TILE);
+        }
         const int2 b = make_int2((int)min(s, (long long)INT_MAX), (int)min(e, (long long)INT_MAX));
-        for (int t = t0 + lane; t <= t1; t += WAVE) {
-            tmap[t] = k;
-            tbound[t] = b;
+        const bool few = t1 - t0 < 2;
+        if (few) {
+            for (int t = t0; t <= t1; t++) {
+                tmap[t] = k;
+                tbound[t] = b;
+            }
+        }
+        u64 many = __ballot(!few);
+        while (many) {
+            const int l = __ffsll(many) - 1;
+            many &= many - 1;
+            const int u0 = __shfl(t0, l, WAVE), u1 = __shfl(t1, l, WAVE);
+            const int2 ub = make_int2(__shfl(b.x, l, WAVE), __shfl(b.y, l, WAVE));
+            for (int t = u0 + lane; t <= u1; t += WAVE) {
+                tmap[t] = k0 + l;
+                tbound[t] = ub;
+            }
         }
     }
 }
@@ -123,8 +148,9 @@ static apsep_status seg_launch(apsep_batch_plan* p, const int* in, const long lo
     TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
     SegIn sg{offs, p->tmap, nseg, p->tbound};
     const int nkey1 = L.nlev > 1 ? sz[1] : 0;
-    // a wave per segment (4 per block), a thread per level-1 key
-    const long long work = nseg * 64ll > nkey1 ? nseg * 64ll : nkey1;
+    // a lane per segment, a thread per level-1 key
+    const long long segs = (nseg + 63ll) / 64 * 64;
+    const long long work = segs > nkey1 ? segs : nkey1;
     int blocks = (int)((work + 255) / 256);
     if (blocks < 1) blocks = 1;
     if (blocks > 4096) blocks = 4096;
