@@ -713,6 +713,15 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     }
     qmask &= vmask;
     omask &= vmask;
+    if (SEG && sl.wallmask) {
+        // an element at or after a segment start in this lane has its
+        // answer, if any, inside the lane: the in-lane one is final (seg_rel
+        // drops it when it lies before the start), so it is neither a
+        // cross-lane query nor open
+        const unsigned from_wall = ~((sl.wallmask & (0u - sl.wallmask)) - 1u);
+        qmask &= ~from_wall;
+        omask &= ~from_wall;
+    }
     // The tile's first element is open (nothing precedes it in the tile).
     // When the key just before the tile (prev_last, read by lane 0)
     // qualifies, that key is its answer: settled here, it leaves the
