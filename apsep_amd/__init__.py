@@ -32,8 +32,7 @@ ALGOS = {"wstl": ALGO_WSTL, "spt": ALGO_SPT, "lookback": ALGO_LOOKBACK,
 
 # every symbol include/apsep/apsep.h declares
 EXPORTS = [
-    "apsep_status_string", "apsep_version", "apsep_last_error",
-    "apsep_cooperative_refusals", "apsep_scratch_create",
+    "apsep_status_string", "apsep_version", "apsep_last_error", "apsep_scratch_create",
     "apsep_scratch_destroy", "apsep_scratch_bytes", "apsep_run",
     "apsep_launch", "apsep_run_mode", "apsep_run_noalloc", "apsep_scratch_query",
     "apsep_scratch_check", "apsep_time_kernels", "apsep_time_kernels_rotating",
@@ -98,8 +97,6 @@ def lib() -> ctypes.CDLL:
         L.apsep_status_string.restype = ctypes.c_char_p
         L.apsep_version.restype = ctypes.c_char_p
         L.apsep_last_error.restype = ctypes.c_char_p
-        L.apsep_cooperative_refusals.argtypes = []
-        L.apsep_cooperative_refusals.restype = i64
         L.apsep_scratch_create.argtypes = [ci, i64, ci, ctypes.POINTER(vp)]
         L.apsep_scratch_destroy.argtypes = [vp]
         L.apsep_scratch_bytes.argtypes = [ci, i64, ci, ctypes.POINTER(i64)]
@@ -140,8 +137,7 @@ def lib() -> ctypes.CDLL:
         L.apsep_server_job_times.argtypes = [vp, i64, i32, vp, vp]
         L.apsep_server_release_upto.argtypes = [vp, i64]
         for f in EXPORTS:
-            if f not in ("apsep_status_string", "apsep_version", "apsep_last_error",
-                         "apsep_cooperative_refusals"):
+            if f not in ("apsep_status_string", "apsep_version", "apsep_last_error"):
                 getattr(L, f).restype = ci
         _lib = L
     return _lib
@@ -589,12 +585,6 @@ def run_noalloc(x, out, n: int | None = None, incl: bool = False, stream=None):
     _check(lib().apsep_run_noalloc(x.data_ptr(), out.data_ptr(), n, int(incl),
                                    stream.cuda_stream), "apsep_run_noalloc")
     return out
-
-
-def cooperative_refusals() -> int:
-    """SPT solves in this process whose cooperative launch the runtime
-    refused and the multi-launch kernels solved instead (same answers)."""
-    return int(lib().apsep_cooperative_refusals())
 
 
 def time_kernels_rotating(scratches, xs, outs, reps: int, n: int | None = None,

@@ -3,8 +3,6 @@
 // Argument checks run before any HIP call, so every error path is testable
 // on a machine without a GPU.  Nothing here prints or exits.
 #include <hip/hip_runtime.h>
-
-#include <atomic>
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
@@ -26,15 +24,6 @@ void record_hip_error(hipError_t e, const char* func, int line) {
              hipGetErrorString(e), func, line);
 }
 
-static std::atomic<long long> g_coop_refusals{0};
-
-void note_cooperative_refusal(hipError_t e) {
-    g_coop_refusals.fetch_add(1, std::memory_order_relaxed);
-    snprintf(g_last_error, sizeof g_last_error,
-             "%s (%s) from hipLaunchCooperativeKernel: solved by the multi-launch kernels",
-             hipGetErrorName(e), hipGetErrorString(e));
-}
-
 static void record_device_error(int code) {
     snprintf(g_last_error, sizeof g_last_error,
              "device-side error word %d: a grid barrier or look-back wait gave up", code);
@@ -45,8 +34,6 @@ static void record_device_error(int code) {
 extern "C" {
 
 const char* apsep_last_error(void) { return g_last_error[0] ? g_last_error : "none"; }
-
-int64_t apsep_cooperative_refusals(void) { return g_coop_refusals.load(std::memory_order_relaxed); }
 
 const char* apsep_status_string(apsep_status s) {
     switch (s) {
@@ -150,7 +137,7 @@ apsep_status apsep_scratch_query(const apsep_scratch* s, apsep_scratch_info* inf
     info->tile_elems = s->tile;
     info->tiles = s->nt_cap;
     info->levels = s->nt_cap > 0 ? level_sizes(s->nt_cap, sz) : 0;
-    info->persistent_blocks = s->coop_grid;
+    info->persistent_blocks = s->spt_grid;
     return APSEP_OK;
 }
 

@@ -1,5 +1,5 @@
 // apsep_device.hpp -- gfx950 device building blocks shared by every APSEP
-// solver variant (WSTL multi-launch, SPT cooperative, segmented).
+// solver variant (WSTL multi-launch, SPT persistent, segmented).
 //
 // Problem (src/apsep.cuh:1-4): out[i] = largest j < i with in[j] < in[i]
 // (strict) or in[j] <= in[i] (INCL), else -1.
@@ -1678,15 +1678,16 @@ __device__ __forceinline__ void fix_tile(const Map& io, int n, int t, const Leve
 }
 
 // ---------------------------------------------------------------------------
-// Grid barrier for persistent cooperative launches.  Arrivals are spread
+// Grid barrier for persistent launches.  Arrivals are spread
 // over 64 sub-counters (one 128-byte line each); the last arriver of a
 // sub-group arrives at the root; the last root arriver bumps the generation
 // word (its own line).  The counters only grow inside a launch (barrier j
 // of a launch is complete for a sub-group at (j + 1) x its size), and a
 // one-workgroup kernel (grid_barrier_reset) zeroes them before every launch,
 // so a launch cut short by a timeout leaves nothing behind for the next.
-// The spin is bounded: a grid that is not fully co-resident (or any other
-// stall) raises `err` and lets every workgroup leave instead of hanging the
+// Only the workgroups that grid_discover found running take part, so every
+// participant reaches every barrier.  The spin is still bounded: any stall
+// raises `err` and lets every workgroup leave instead of hanging the
 // device; `err` stays set until the host reads it (apsep_scratch_check).
 // ---------------------------------------------------------------------------
 constexpr int BAR_SUB = 64;
@@ -1698,16 +1699,72 @@ struct GridBarrier {
     unsigned pad0[BAR_LINE - 2];
     unsigned root;
     unsigned pad1[BAR_LINE - 1];
+    unsigned disc;    // discovery tickets taken | DISC_CLOSED
+    unsigned pad2[BAR_LINE - 1];
+    unsigned pcount;  // participants of the launch, published by the first
+    unsigned pad3[BAR_LINE - 1];
     unsigned sub[BAR_SUB * BAR_LINE];
 };
 
 constexpr unsigned BARRIER_SPIN_LIMIT = 1u << 22;  // >= 1 s at >= 250 ns per poll
+constexpr unsigned DISC_CLOSED = 1u << 31;
+// how long the first workgroup waits for the others to start: 50 us of the
+// 100 MHz real-time clock (all of them start within a few us on an idle GPU)
+constexpr unsigned long long DISC_WAIT_TICKS = 5000;
 
 // Launched (one workgroup of BAR_SUB lanes) before each persistent launch.
 __device__ __forceinline__ void grid_barrier_reset(GridBarrier* b) {
     const int i = (int)threadIdx.x;
     if (i < BAR_SUB) b->sub[i * BAR_LINE] = 0u;
-    if (i == 0) b->root = 0u;
+    if (i == 0) {
+        b->root = 0u;
+        b->disc = 0u;
+        b->pcount = 0u;
+    }
+}
+
+// Which workgroups of a normally launched persistent grid run at the same
+// time.  Each workgroup takes a ticket; the first one waits (bounded in
+// time) until every workgroup of the grid has taken one, then closes the
+// count and publishes it.  A workgroup that took its ticket before the
+// close is running, so it is a participant and will reach every barrier;
+// one that arrives after the close (it could not start while the others
+// held the GPU) leaves at once.  Returns (participant index, participant
+// count), or (-1, 0) for a workgroup that leaves; uniform per workgroup.
+__device__ __forceinline__ int2 grid_discover(GridBarrier* b) {
+    __shared__ int2 s_r;
+    if (threadIdx.x == 0) {
+        const unsigned old =
+            __hip_atomic_fetch_add(&b->disc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int id = (old & DISC_CLOSED) ? -1 : (int)old;
+        int np = 0;
+        if (id == 0) {
+            const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+            while ((__hip_atomic_load(&b->disc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+                    ~DISC_CLOSED) < gridDim.x &&
+                   __builtin_amdgcn_s_memrealtime() - t0 < DISC_WAIT_TICKS)
+                __builtin_amdgcn_s_sleep(1);
+            np = (int)(__hip_atomic_fetch_or(&b->disc, DISC_CLOSED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT) &
+                       ~DISC_CLOSED);
+            __hip_atomic_store(&b->pcount, (unsigned)np, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else if (id > 0) {
+            unsigned spins = 0, pc;
+            while ((pc = __hip_atomic_load(&b->pcount, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT)) == 0u) {
+                if (++spins > BARRIER_SPIN_LIMIT) {
+                    __hip_atomic_store(&b->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            np = (int)pc;
+            if (np == 0) id = -1;
+        }
+        s_r = make_int2(id, np);
+    }
+    __syncthreads();
+    return s_r;
 }
 
 // Generation a launch starts from.  The counter persists across launches;
@@ -1726,14 +1783,15 @@ __device__ __forceinline__ unsigned grid_barrier_start(GridBarrier* b) {
 // loads read the other workgroups' words from the coherent level.  `leader`
 // is true for exactly one lane of the workgroup.  Returns false (uniformly
 // per workgroup) if the barrier timed out.
-__device__ __forceinline__ bool grid_barrier_co(GridBarrier* b, unsigned nblocks,
+// `me`: the workgroup's participant index, `nblocks`: the participant count.
+__device__ __forceinline__ bool grid_barrier_co(GridBarrier* b, unsigned nblocks, unsigned me,
                                                 unsigned& gen, unsigned& j, bool leader) {
     __shared__ int s_ok;
     wait_own_memory_ops();
     __syncthreads();
     if (leader) {
         int ok = 1;
-        const unsigned g = blockIdx.x % BAR_SUB;
+        const unsigned g = me % BAR_SUB;
         const unsigned nsub = nblocks < (unsigned)BAR_SUB ? nblocks : (unsigned)BAR_SUB;
         const unsigned gsz = nblocks / BAR_SUB + (g < nblocks % BAR_SUB ? 1u : 0u);
         bool release = false;

@@ -26,7 +26,7 @@ struct apsep_scratch {
     int2* qlist;
     int* scount;
     int2* spine;
-    int coop_grid;        // SPT persistent grid size (occupancy x CUs)
+    int spt_grid;        // SPT persistent grid size (occupancy x CUs)
     apsep::GridBarrier* bar;
     // look-back runs: control words and the published (tagged) words
     apsep::LbCtl* lb_ctl;
@@ -84,9 +84,6 @@ inline hipError_t take_hip_error(const char* func, int line) {
     return e;
 }
 #define APSEP_CLEAR_HIP_ERROR() ((void)::apsep::take_hip_error(__func__, __LINE__))
-// Counts a cooperative launch the runtime refused and the library replaced
-// by the multi-launch kernels (apsep_cooperative_refusals reports the count).
-void note_cooperative_refusal(hipError_t e);
 // true when the launches just issued failed (the error is recorded)
 #define APSEP_LAUNCH_FAILED() (::apsep::take_hip_error(__func__, __LINE__) != hipSuccess)
 

@@ -3,8 +3,9 @@
 //
 //   WSTL     tile pass -> level pass (64-ary tile-min hierarchy) -> fix-up
 //            pass (multi-launch; replaces runWSTL, src/apsep.cuh:2548-2596)
-//   SPT      the same phases in one cooperative persistent launch with two
-//            grid barriers (replaces runSPT, src/apsep.cuh:3052-3067)
+//   SPT      the same phases in one persistent launch with two grid
+//            barriers among the workgroups that run together (replaces
+//            runSPT, src/apsep.cuh:3052-3067)
 //
 // All device work is integer compare/min and index arithmetic; no MFMA.
 #include <hip/hip_runtime.h>
@@ -70,31 +71,35 @@ __global__ __launch_bounds__(FIX_WG) __attribute__((amdgpu_waves_per_eu(8))) voi
     fix_tile<INCL>(FlatMap{in, out}, n, t, L, to, lds[threadIdx.x >> 6]);
 }
 
-// SPT: one persistent launch of fully co-resident workgroups (launched
-// cooperatively), three per CU: the tile phase over tiles blockIdx.x,
-// blockIdx.x + grid, ..., a grid barrier, [upper levels by workgroup 0, a
-// grid barrier,] then the fix-up of the same tiles, one wave per tile.  A
-// tile is fixed by the workgroup that solved it, so its outputs and open
-// list stay in one XCD's L2; the summaries other workgroups read (tile
-// mins, chunk mins, spines, counts, upper keys) are written and read with
-// device-coherent accesses, and the barrier needs no fence.  The occupancy
-// attribute guarantees SPT_WAVES_PER_EU waves per SIMD and the host never
-// launches more workgroups per CU, so the grid is co-resident whatever
-// register budget the compiler picks.  The fix-up's LDS reuses the tile's.
+// SPT: one persistent launch, normally launched with at most as many
+// workgroups as fit on the GPU at once (three per CU).  grid_discover finds
+// the workgroups that actually run together (normally all of them); each
+// participant solves tiles me, me + np, ..., then a grid barrier among the
+// participants, [upper levels by participant 0, a barrier,] then the fix-up
+// of the same tiles, one wave per tile.  A tile is fixed by the workgroup
+// that solved it, so its outputs and open list stay in one XCD's L2; the
+// summaries other workgroups read (tile mins, chunk mins, spines, counts,
+// upper keys) are written and read with device-coherent accesses, and the
+// barrier needs no fence.  The occupancy attribute guarantees
+// SPT_WAVES_PER_EU waves per SIMD.  The fix-up's LDS reuses the tile's.
+// (A cooperative launch, which has the runtime guarantee co-residency
+// instead, costs ~17 us more per launch on MI355X.)
 template <bool INCL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SPT_WAVES_PER_EU)))
 void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
            TileOut to, Levels L, GridBarrier* bar) {
     __shared__ TileSmem sm;
     static_assert(NWAVE * FIX_LDS_INTS * sizeof(int) <= sizeof(TileSmem), "fix-up LDS");
+    const int2 pr = grid_discover(bar);
+    const int me = pr.x, np = pr.y;
+    if (me < 0) return;  // uniform per workgroup: not a participant
     // Nothing per-lane stays live across the tile loop (it would be spilled
     // to memory): the wave index is kept in a scalar, and each tile computes
     // the lane's index afresh inside a volatile asm, so the compiler cannot
     // hoist the lane-derived addresses out of the loop.
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int nt = L.nt;
-    const int grid = (int)gridDim.x;
-    for (int t = (int)blockIdx.x; t < nt; t += grid) {
+    for (int t = me; t < nt; t += np) {
         int lane;
         asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0"
                      : "=v"(lane));
@@ -103,14 +108,14 @@ void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
     if (nt <= 1) return;  // uniform across the grid
     const bool leader = wave == 0 && lane_id() == 0;
     unsigned gen = grid_barrier_start(bar), j = 0;  // before this workgroup arrives
-    if (!grid_barrier_co(bar, gridDim.x, gen, j, leader)) return;
+    if (!grid_barrier_co(bar, (unsigned)np, (unsigned)me, gen, j, leader)) return;
     if (L.upper) {
-        if (blockIdx.x == 0) build_upper_levels<true>(L, wave);
-        if (!grid_barrier_co(bar, gridDim.x, gen, j, leader)) return;
+        if (me == 0) build_upper_levels<true>(L, wave);
+        if (!grid_barrier_co(bar, (unsigned)np, (unsigned)me, gen, j, leader)) return;
     }
     int* lds = reinterpret_cast<int*>(&sm) + wave * FIX_LDS_INTS;
     for (int k = wave;; k += NWAVE) {
-        const long long t = (long long)blockIdx.x + (long long)k * grid;
+        const long long t = (long long)me + (long long)k * np;
         if (t >= nt) break;  // uniform per wave
         fix_tile<INCL, false, true>(FlatMap{in, out}, n, (int)t, L, to, lds);
     }
@@ -268,12 +273,7 @@ apsep_status single_scratch_init(apsep_scratch* s) {
             APSEP_CLEAR_HIP_ERROR();
             return APSEP_ERR_UNSUPPORTED;
         }
-        int coop = 0;
-        if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, s->device) != hipSuccess || !coop) {
-            APSEP_CLEAR_HIP_ERROR();
-            return APSEP_ERR_UNSUPPORTED;
-        }
-        s->coop_grid = per_cu * s->num_cus;
+        s->spt_grid = per_cu * s->num_cus;
     }
     return APSEP_OK;
 }
@@ -351,34 +351,18 @@ static apsep_status run_wstl(apsep_scratch* s, const int* in, int* out, int n,
 template <bool INCL>
 static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
                             hipStream_t st) {
-    // a cooperative launch is not recorded into graphs
-    if (capture_state(st) != 0) return APSEP_ERR_UNSUPPORTED;
+    const int cap = capture_state(st);
+    if (cap < 0) return APSEP_ERR_UNSUPPORTED;
     const int nt = (int)(((long long)n + TILE - 1) / TILE);  // (n + TILE - 1 overflows near 2^31)
     Levels L;
-    if (fill_levels(s, nt, L, st) != APSEP_OK) return APSEP_ERR_LAUNCH;
+    if (fill_levels(s, nt, L, st, cap) != APSEP_OK) return APSEP_ERR_LAUNCH;
     TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
-    int grid = s->coop_grid < nt ? s->coop_grid : nt;
+    int grid = s->spt_grid < nt ? s->spt_grid : nt;
     if (grid < 1) grid = 1;
     GridBarrier* bar = s->bar;
     hipLaunchKernelGGL(k_spt_begin, dim3(1), dim3(BAR_SUB), 0, st, bar);
-    if (APSEP_LAUNCH_FAILED()) return APSEP_ERR_LAUNCH;
-    void* args[] = {(void*)&in, (void*)&out, (void*)&n, (void*)&to, (void*)&L, (void*)&bar};
-    hipError_t e = hipLaunchCooperativeKernel((const void*)k_spt<INCL>, dim3(grid), dim3(NT),
-                                              args, 0, st);
-    if (e == hipErrorIllegalState) {
-        // The runtime refused the cooperative launch and nothing ran (seen
-        // now and then in a process making many cooperative launches while
-        // another process on the device also made some): the multi-launch
-        // kernels solve on the same scratch instead, with the same answers.
-        (void)hipGetLastError();
-        note_cooperative_refusal(e);
-        return run_wstl<INCL>(s, in, out, n, st);
-    }
-    if (e != hipSuccess) {
-        APSEP_CLEAR_HIP_ERROR();
-        return APSEP_ERR_LAUNCH;
-    }
-    return APSEP_OK;
+    hipLaunchKernelGGL(k_spt<INCL>, dim3(grid), dim3(NT), 0, st, in, out, n, to, L, bar);
+    return APSEP_LAUNCH_FAILED() ? APSEP_ERR_LAUNCH : APSEP_OK;
 }
 
 // Launch r of each kernel solves array r % m (its own scratch, input and

@@ -35,13 +35,14 @@ typedef enum apsep_status {
     APSEP_ERR_LAUNCH = 5,       /* kernel launch / runtime call failed      */
     APSEP_ERR_DEVICE = 6,       /* no usable HIP device, or wrong device    */
     APSEP_ERR_TOO_LARGE = 7,    /* n > INT32_MAX (int32 indices)            */
-    APSEP_ERR_UNSUPPORTED = 8,  /* e.g. cooperative launch not supported    */
+    APSEP_ERR_UNSUPPORTED = 8,  /* e.g. a stream in an invalidated capture  */
     APSEP_ERR_BUDGET = 9        /* serving: job exceeds the memory budget   */
 } apsep_status;
 
 /* Solver variants.  WSTL = multi-launch pipeline (tile pass, 64-ary tree
- * build, fix-up pass); SPT = one cooperative persistent launch with grid
- * barriers; LOOKBACK = one non-cooperative launch with decoupled look-back;
+ * build, fix-up pass); SPT = one persistent launch with grid barriers
+ * among the workgroups that run together; LOOKBACK = one launch with
+ * decoupled look-back;
  * AUTO = the fastest measured for the size. */
 typedef enum apsep_algo {
     APSEP_ALGO_WSTL = 0,
@@ -66,12 +67,6 @@ const char* apsep_version(void);
  * the next failure. */
 const char* apsep_last_error(void);
 
-/* No counterpart in the reference; diagnostics.  The number of SPT solves
- * in this process whose cooperative launch the HIP runtime refused
- * (hipErrorIllegalState: nothing ran) and that the multi-launch kernels
- * solved instead, on the same scratch and stream, with the same answers. */
-int64_t apsep_cooperative_refusals(void);
-
 /* Replaces allocWSTLScratch (src/apsep.cuh:2520-2537) and allocSPTScratch
  * (src/apsep.cuh:3013-3040): device scratch sized for up to n_cap elements,
  * bound to the current device.  n_cap may be 0 (then only n = 0 runs). */
@@ -90,7 +85,11 @@ apsep_status apsep_scratch_bytes(apsep_algo algo, int64_t n_cap, int incl,
  * (src/apsep.cuh:3052-3067): asynchronous solve on `stream`.
  * n <= 0 is a no-op returning APSEP_OK (as the reference returns early);
  * n > capacity -> APSEP_ERR_CAPACITY.  d_in and d_out: n int32 each,
- * 16-byte aligned, not overlapping.  d_out needs no initialisation. */
+ * 16-byte aligned, not overlapping.  d_out needs no initialisation.
+ * May be recorded into a graph (stream capture) and replayed: a captured
+ * run takes a fixed epoch and empties its level-1 keys with a kernel, so
+ * captured and eager runs share a scratch in any order; a stream in an
+ * invalidated capture state gives APSEP_ERR_UNSUPPORTED. */
 apsep_status apsep_run(apsep_scratch* s, const int32_t* d_in, int32_t* d_out,
                        int64_t n, void* stream);
 
@@ -133,7 +132,7 @@ typedef struct apsep_scratch_info {
     int32_t tile_elems;        /* elements per tile (workgroup)             */
     int32_t tiles;             /* tiles at capacity (num_blocks)            */
     int32_t levels;            /* levels of the 64-ary tile-min hierarchy   */
-    int32_t persistent_blocks; /* SPT cooperative grid (num_phys), else 0   */
+    int32_t persistent_blocks; /* SPT persistent grid (num_phys), else 0    */
     int32_t reserved;
 } apsep_scratch_info;
 

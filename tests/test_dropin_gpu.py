@@ -2,16 +2,16 @@
 
 * tests/dropin/dropin_caller: a C++ program built by hipcc that includes
   only include/apsep/apsep.hpp and links libapsep.so (built twice: the
-  shims as shipped, and with the SPT names forced onto the cooperative
-  kernel, tests/dropin/dropin_caller_spt).  It calls
+  shims as shipped, and with the SPT names forced onto the single
+  persistent kernel, tests/dropin/dropin_caller_spt).  It calls
   launchWSTL / launchSPT (a device allocation per case, as the reference's
   src/main.cu:228-252 does), allocSPTScratch<int,128,4,INCL> + runSPT and
   allocWSTLScratch + runWSTL on one scratch for every case (as
   src/bench_spt_le.cu:60-91 does), and free*Scratch at the end.
 * apsep_launch (the C ABI under launchWSTL / launchSPT) called directly for
   every algorithm and both predicates.
-* apsep_run under graph capture: the WSTL sequence is recorded and replayed
-  with new keys; SPT (a cooperative launch) refuses the capture.
+* apsep_run under graph capture: the WSTL sequence and the SPT launch are
+  recorded and replayed with new keys.
 """
 import os
 import subprocess
@@ -26,7 +26,7 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # the shims as shipped (the SPT names run the fastest solver), and built with
-# APSEP_SHIM_SPT_ALGO=APSEP_ALGO_SPT (the SPT names run the cooperative kernel)
+# APSEP_SHIM_SPT_ALGO=APSEP_ALGO_SPT (the SPT names run the persistent kernel)
 CALLERS = {"default": os.path.join(HERE, "dropin", "dropin_caller"),
            "spt": os.path.join(HERE, "dropin", "dropin_caller_spt")}
 POISON = np.int32(np.uint32(0xABABABAB).view(np.int32))
@@ -172,11 +172,14 @@ def test_apsep_launch_on_data(torch_mod, algo):
             assert np.array_equal(got, exp), (algo, label, incl)
 
 
-def test_wstl_graph_capture_and_replay(torch_mod):
-    """apsep_run on a WSTL scratch recorded into a graph (torch.cuda.graph):
-    the run resets level 1 itself and uses the capture epoch, so replays with
-    new keys in the same buffer are exact, with eager runs on the same
-    scratch between them (their epochs interleave with the captured one)."""
+@pytest.mark.parametrize("algo", ["wstl", "spt"])
+def test_graph_capture_and_replay(torch_mod, algo):
+    """apsep_run on a WSTL or SPT scratch recorded into a graph
+    (torch.cuda.graph): the run resets level 1 itself and uses the capture
+    epoch (SPT also resets its barrier and discovery words with a kernel),
+    so replays with new keys in the same buffer are exact, with eager runs
+    on the same scratch between them (their epochs interleave with the
+    captured one)."""
     import apsep_amd as A
     torch = torch_mod
     n = (1 << 22) + 12345
@@ -185,7 +188,7 @@ def test_wstl_graph_capture_and_replay(torch_mod):
     x2 = torch.from_numpy(oracle.fill("random", n, seed=99)).cuda()
     out2 = torch.empty(n, dtype=torch.int32, device="cuda")
     exp2 = oracle.apsep(x2.cpu().numpy())
-    with A.Scratch("wstl", n) as s:
+    with A.Scratch(algo, n) as s:
         x.copy_(torch.from_numpy(oracle.fill("random", n, seed=1)))
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
@@ -202,7 +205,7 @@ def test_wstl_graph_capture_and_replay(torch_mod):
             x.copy_(torch.from_numpy(a))
             out.fill_(int(POISON))
             torch.cuda.synchronize()
-            _log("wstl graph replay", rep, kind)
+            _log(algo, "graph replay", rep, kind)
             g.replay()
             torch.cuda.synchronize()
             s.check()
@@ -214,30 +217,3 @@ def test_wstl_graph_capture_and_replay(torch_mod):
             s.run(x2, out2)
             torch.cuda.synchronize()
             assert np.array_equal(out2.cpu().numpy(), exp2), ("eager after replay", rep)
-
-
-def test_spt_refuses_graph_capture(torch_mod):
-    """A cooperative launch is not recorded into graphs: apsep_run on an SPT
-    scratch under capture returns APSEP_ERR_UNSUPPORTED and launches
-    nothing; the same scratch then runs eagerly."""
-    import apsep_amd as A
-    torch = torch_mod
-    n = 100000
-    a = oracle.fill("random", n, seed=3)
-    x = torch.from_numpy(a).cuda()
-    out = torch.full((n,), int(POISON), dtype=torch.int32, device="cuda")
-    with A.Scratch("spt", n) as s:
-        g = torch.cuda.CUDAGraph()
-        status = None
-        with torch.cuda.graph(g):
-            out.zero_()  # keep the captured graph non-empty
-            try:
-                s.run(x, out)
-            except A.ApsepError as e:
-                status = e.status
-        assert status == A.ERR_UNSUPPORTED
-        torch.cuda.synchronize()
-        s.run(x, out)
-        torch.cuda.synchronize()
-        s.check()
-        assert np.array_equal(out.cpu().numpy(), oracle.apsep(a))
