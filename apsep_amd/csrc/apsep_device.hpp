@@ -266,10 +266,6 @@ __device__ __forceinline__ int tile_value(const TileSmem& sm, int e) {
     return w[((e >> 2) & (K4 - 1)) * (NT * 4) + (e >> 4) * 4 + (e & 3)];
 }
 
-__device__ __forceinline__ int wave_shfl_up(int x, int d) {
-    return __shfl_up(x, d, WAVE);
-}
-
 // DPP data-parallel-primitive controls (GFX9 encoding): a VALU operand read
 // from another lane of the same row (16 lanes) or wave, at no extra latency.
 constexpr int DPP_ROW_SHL = 0x100;      // + n: lane + n inside the row
@@ -298,14 +294,6 @@ __device__ __forceinline__ int lane_id() {
 }
 __device__ __forceinline__ int lane63(int x) { return __builtin_amdgcn_readlane(x, WAVE - 1); }
 
-// Inclusive min scan over the wave (Hillis-Steele), lanes below the shift
-// keep their value, so every step is a window clipped at lane 0.
-__device__ __forceinline__ int wave_incl_min(int x) {
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) x = min(x, wave_shfl_up(x, d));
-    return x;
-}
-
 // Min over the wave (DPP row scans, then the row carries; no LDS round
 // trip).  The whole wave must be active.
 __device__ __forceinline__ int wave_min(int x) {
@@ -325,32 +313,6 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// Rightmost index in [lo, hi] with qual(arr[idx], v), or -1.  Reads aligned
-// dwordx4 groups; arr must be readable (padded) up to (hi | 3).
-template <bool INCL, bool CO = false>
-__device__ __forceinline__ int scan_back(const int* __restrict__ arr, int lo,
-                                         int hi, int v) {
-    int i = hi;
-    while (i >= lo) {
-        const int a4 = i & ~3;
-        const int4 w = ld_i4<CO>(reinterpret_cast<const int4*>(arr + a4));
-        if (a4 + 3 <= i && a4 + 3 >= lo && qual<INCL>(w.w, v)) return a4 + 3;
-        if (a4 + 2 <= i && a4 + 2 >= lo && qual<INCL>(w.z, v)) return a4 + 2;
-        if (a4 + 1 <= i && a4 + 1 >= lo && qual<INCL>(w.y, v)) return a4 + 1;
-        if (a4 >= lo && qual<INCL>(w.x, v)) return a4;
-        i = a4 - 1;
-    }
-    return -1;
-}
-
-// Same over an epoch-keyed level.
-template <bool INCL, bool CO = false>
-__device__ __forceinline__ int scan_back_key(const Levels& L, int lvl, int lo, int hi, int v) {
-    for (int i = hi; i >= lo; i--)
-        if (qual<INCL>(key_value(ld_u64<CO>(&key_at(L, lvl, i)), L.epoch), v)) return i;
-    return -1;
-}
-
 // Rightmost qualifying index in the aligned 32-entry block [b, b+32) (b a
 // multiple of 4), all loads issued together; -1 if none.
 template <bool INCL, bool CO = false>
@@ -367,37 +329,6 @@ __device__ __forceinline__ int last_in_32(const int* __restrict__ arr, int b, in
         r = qual<INCL>(w[k].w, v) ? b + 4 * k + 3 : r;
     }
     return r;
-}
-
-// Nearest tile s < t with qual(tile min, v); the caller guarantees one
-// exists (some tile before t qualifies).  Ascends the 64-ary hierarchy until an
-// earlier entry qualifies, then descends to the tile.
-template <bool INCL, bool CO = false>
-__device__ int find_tile(const Levels& L, int t, int v) {
-    int found = scan_back<INCL, CO>(L.mn0, (t - 1) & ~(FAN - 1), t - 1, v);
-    if (found >= 0) return found;
-    int x = (t - 1) >> LOG_FAN;  // exclusive bound at level 1
-    if (L.nlev < 2 || x <= 0) return -1;  // unreachable when guaranteed
-    int lvl = 1;
-    if (L.upper) {
-        while (true) {
-            found = scan_back_key<INCL, CO>(L, lvl, (x - 1) & ~(FAN - 1), x - 1, v);
-            if (found >= 0) break;
-            x = (x - 1) >> LOG_FAN;
-            lvl++;
-            if (lvl >= L.nlev || x <= 0) return -1;
-        }
-        while (lvl > 1) {
-            lvl--;
-            const int lo = found << LOG_FAN;
-            found = scan_back_key<INCL, CO>(L, lvl, lo, min(lo + FAN - 1, L.n[lvl] - 1), v);
-        }
-    } else {
-        found = scan_back_key<INCL, CO>(L, 1, 0, x - 1, v);
-        if (found < 0) return -1;
-    }
-    const int lo = found << LOG_FAN;
-    return scan_back<INCL, CO>(L.mn0, lo, min(lo + FAN - 1, L.n[0] - 1), v);
 }
 
 // Rightmost j in [0, K) with qual(lane lt's value j, v), or -1.
@@ -1096,9 +1027,10 @@ __device__ __forceinline__ void build_upper_levels(const Levels& L, int wv) {
     }
 }
 
-// Wave-cooperative version of find_tile for one query value v (uniform):
-// every level step is one coalesced 64-entry load and a ballot, so an answer
-// arbitrarily far back costs a few rounds instead of a per-lane walk.
+// find_tile_wave (below): the nearest tile before tb whose min qualifies
+// for v (uniform), found by the whole wave.  Every level step is one
+// coalesced 64-entry load and a ballot, so an answer arbitrarily far back
+// costs a few rounds instead of a per-lane walk.
 template <bool CO = false>
 __device__ __forceinline__ int level_entry(const Levels& L, int lvl, int idx) {
     return lvl == 0 ? ld_i<CO>(&L.mn0[idx]) : key_value(ld_u64<CO>(&key_at(L, lvl, idx)), L.epoch);
