@@ -470,7 +470,10 @@ class Server:
 
     def __init__(self, pool_bytes: int, max_batch: int, max_batch_jobs: int = 4096,
                  incl: bool = False):
+        import torch
         self._h = ctypes.c_void_p(None)
+        # the server is bound to the current device (as apsep_server_create)
+        self.device = torch.cuda.current_device()
         _check(lib().apsep_server_create(pool_bytes, max_batch, max_batch_jobs, int(incl),
                                          ctypes.byref(self._h)), "apsep_server_create")
 
@@ -504,13 +507,13 @@ class Server:
         if info.state != JOB_DONE:
             raise ApsepError(ERR_INVALID_ARG, "job %d is not done" % job_id)
         if out is None:
-            out = torch.empty(info.n, dtype=torch.int32, device="cuda")
+            out = torch.empty(info.n, dtype=torch.int32, device=f"cuda:{self.device}")
         if out.dtype != torch.int32 or not out.is_cuda or not out.is_contiguous() \
-                or out.numel() < info.n:
-            raise ValueError("apsep: answers need a contiguous int32 device tensor of "
-                             ">= %d elements" % info.n)
+                or out.numel() < info.n or _device_index(out) != self.device:
+            raise ValueError("apsep: answers need a contiguous int32 tensor of >= %d "
+                             "elements on the server's device" % info.n)
         if info.n:
-            cs = torch.cuda.current_stream()
+            cs = torch.cuda.current_stream(out.device)
             if _hip().hipMemcpyAsync(out.data_ptr(), info.d_answers, info.n * 4,
                                      3, cs.cuda_stream) != 0:  # hipMemcpyDeviceToDevice
                 raise ApsepError(ERR_LAUNCH, "hipMemcpyAsync")
