@@ -528,7 +528,8 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
                                            int prev_last, TileSmem& sm, const TileOut& to,
                                            const Levels& L,
                                            const SegLane sl = SegLane{0, 0u},
-                                           const int tid = (int)threadIdx.x) {
+                                           const int tid = (int)threadIdx.x,
+                                           unsigned* tk_ctr = nullptr, unsigned* tk_slot = nullptr) {
     const int lane = tid & (WAVE - 1);
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar tests
     const int row = lane >> 4;
@@ -596,6 +597,12 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         sm.wmin[wave] = min(rm0, r123);
     }
     __syncthreads();
+    // a persistent solve's ticket for its next tile (tk_ctr): taken once the
+    // tile's keys have arrived and handed over (*tk_slot) before the tile's
+    // closing barrier, so its round trip overlaps the tile's work
+    unsigned tk = 0u;
+    if (tk_ctr && tid == 0)
+        tk = __hip_atomic_fetch_add(tk_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
     // mins of the earlier and of the later waves: uniform, so they are
     // folded in scalar registers
@@ -925,6 +932,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         st_i<CO>(&to.qcount[t], total);
         st_i<CO>(&to.scount[t], stotal);
         if (L1 && L.nlev > 1) atomicMin(&key_at(L, 1, t >> LOG_FAN), make_key(L.epoch, m));
+        if (tk_ctr) *tk_slot = tk;
     }
     __syncthreads();  // sm is reused by the next tile
 }
@@ -1617,7 +1625,7 @@ __device__ __forceinline__ void fix_tile(const Map& io, int n, int t, const Leve
 // of a launch is complete for a sub-group at (j + 1) x its size), and a
 // one-workgroup kernel (grid_barrier_reset) zeroes them before every launch,
 // so a launch cut short by a timeout leaves nothing behind for the next.
-// Only the workgroups that grid_discover found running take part, so every
+// Only the workgroups that grid_join found running take part, so every
 // participant reaches every barrier.  The spin is still bounded: any stall
 // raises `err` and lets every workgroup leave instead of hanging the
 // device; `err` stays set until the host reads it (apsep_scratch_check).
@@ -1625,78 +1633,110 @@ __device__ __forceinline__ void fix_tile(const Map& io, int n, int t, const Leve
 constexpr int BAR_SUB = 64;
 constexpr int BAR_LINE = 32;  // uints per 128-byte line
 
+// Workgroups of a persistent launch join through SPT_SLOTS counters, one
+// per residue of blockIdx.x, each on its own line: a device atomic on one
+// address serialises (~13 ns each on MI355X), so 768 joins on one counter
+// kept the last workgroup waiting ~10 us.  Slot c's participants work on
+// the tiles of residue c (see k_spt).
+constexpr int SPT_SLOTS = 8;
+constexpr int SPT_SLOT_MAX = 255;  // participants per slot (a byte of the published word)
+
 struct GridBarrier {
     unsigned gen;
     int err;
     unsigned pad0[BAR_LINE - 2];
     unsigned root;
     unsigned pad1[BAR_LINE - 1];
-    unsigned disc;    // discovery tickets taken | DISC_CLOSED
-    unsigned pad2[BAR_LINE - 1];
-    unsigned pcount;  // participants of the launch, published by the first
-    unsigned pad3[BAR_LINE - 1];
+    u64 pub;          // participants per slot, one byte each (0: not yet published)
+    unsigned pad2[BAR_LINE - 2];
+    unsigned disc[SPT_SLOTS * BAR_LINE];   // per slot: joins | DISC_CLOSED
+    unsigned tiles[SPT_SLOTS * BAR_LINE];  // per slot: tickets of dynamically taken tiles
     unsigned sub[BAR_SUB * BAR_LINE];
 };
 
 constexpr unsigned BARRIER_SPIN_LIMIT = 1u << 22;  // >= 1 s at >= 250 ns per poll
 constexpr unsigned DISC_CLOSED = 1u << 31;
-// how long the first workgroup waits for the others to start: 50 us of the
-// 100 MHz real-time clock (all of them start within a few us on an idle GPU)
+// how long the closing workgroup waits for the others to start: 50 us of
+// the 100 MHz real-time clock (all of them start within a few us on an
+// idle GPU)
 constexpr unsigned long long DISC_WAIT_TICKS = 5000;
 
 // Launched (one workgroup of BAR_SUB lanes) before each persistent launch.
 __device__ __forceinline__ void grid_barrier_reset(GridBarrier* b) {
     const int i = (int)threadIdx.x;
     if (i < BAR_SUB) b->sub[i * BAR_LINE] = 0u;
+    if (i < SPT_SLOTS) {
+        b->disc[i * BAR_LINE] = 0u;
+        b->tiles[i * BAR_LINE] = 0u;
+    }
     if (i == 0) {
         b->root = 0u;
-        b->disc = 0u;
-        b->pcount = 0u;
+        b->pub = 0ull;
     }
 }
 
 // Which workgroups of a normally launched persistent grid run at the same
-// time.  Each workgroup takes a ticket; the first one waits (bounded in
-// time) until every workgroup of the grid has taken one, then closes the
-// count and publishes it.  A workgroup that took its ticket before the
-// close is running, so it is a participant and will reach every barrier;
-// one that arrives after the close (it could not start while the others
-// held the GPU) leaves at once.  Returns (participant index, participant
-// count), or (-1, 0) for a workgroup that leaves; uniform per workgroup.
-__device__ __forceinline__ int2 grid_discover(GridBarrier* b) {
-    __shared__ int2 s_r;
+// time.  Each workgroup takes a ticket from its slot's counter (grid_join);
+// one that arrives after its slot was closed (it could not start while the
+// others held the GPU) leaves at once.  A workgroup that took its ticket
+// before the close is running, so it is a participant and will reach every
+// barrier; slot c's tickets 0 .. npc-1 go to its participants.  The slots
+// are closed by the workgroup holding slot 0's ticket 0 (grid_count),
+// which first waits, bounded in time, until every workgroup of the grid has
+// taken a ticket, then publishes the per-slot counts in one word.
+// Participants may work before they need the counts: the first tile of a
+// persistent solve follows from the ticket alone.  Uniform per workgroup.
+__device__ __forceinline__ int grid_join(GridBarrier* b) {
+    __shared__ int s_id;
     if (threadIdx.x == 0) {
-        const unsigned old =
-            __hip_atomic_fetch_add(&b->disc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int id = (old & DISC_CLOSED) ? -1 : (int)old;
-        int np = 0;
-        if (id == 0) {
+        const unsigned c = blockIdx.x % SPT_SLOTS;
+        const unsigned old = __hip_atomic_fetch_add(&b->disc[c * BAR_LINE], 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        s_id = (old & DISC_CLOSED) ? -1 : (int)old;
+    }
+    __syncthreads();
+    return s_id;
+}
+
+// The published per-slot participant counts (0 after a timeout: the caller
+// leaves).  `closer`: slot 0's ticket 0.
+__device__ __forceinline__ u64 grid_count(GridBarrier* b, bool closer) {
+    __shared__ u64 s_pub;
+    if (threadIdx.x == 0) {
+        u64 pub = 0ull;
+        if (closer) {
             const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-            while ((__hip_atomic_load(&b->disc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
-                    ~DISC_CLOSED) < gridDim.x &&
-                   __builtin_amdgcn_s_memrealtime() - t0 < DISC_WAIT_TICKS)
+            for (;;) {
+                unsigned joined = 0u;
+                for (int c = 0; c < SPT_SLOTS; c++)
+                    joined += __hip_atomic_load(&b->disc[c * BAR_LINE], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT) & ~DISC_CLOSED;
+                if (joined >= gridDim.x || __builtin_amdgcn_s_memrealtime() - t0 >= DISC_WAIT_TICKS)
+                    break;
                 __builtin_amdgcn_s_sleep(1);
-            np = (int)(__hip_atomic_fetch_or(&b->disc, DISC_CLOSED, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT) &
-                       ~DISC_CLOSED);
-            __hip_atomic_store(&b->pcount, (unsigned)np, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else if (id > 0) {
-            unsigned spins = 0, pc;
-            while ((pc = __hip_atomic_load(&b->pcount, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT)) == 0u) {
+            }
+            for (int c = 0; c < SPT_SLOTS; c++) {
+                const unsigned cnt = __hip_atomic_fetch_or(&b->disc[c * BAR_LINE], DISC_CLOSED,
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+                                     ~DISC_CLOSED;
+                pub |= (u64)min(cnt, (unsigned)SPT_SLOT_MAX) << (8 * c);
+            }
+            __hip_atomic_store(&b->pub, pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned spins = 0;
+            while ((pub = __hip_atomic_load(&b->pub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ==
+                   0ull) {
                 if (++spins > BARRIER_SPIN_LIMIT) {
                     __hip_atomic_store(&b->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(2);
             }
-            np = (int)pc;
-            if (np == 0) id = -1;
         }
-        s_r = make_int2(id, np);
+        s_pub = pub;
     }
     __syncthreads();
-    return s_r;
+    return s_pub;
 }
 
 // Generation a launch starts from.  The counter persists across launches;

@@ -72,52 +72,123 @@ __global__ __launch_bounds__(FIX_WG) __attribute__((amdgpu_waves_per_eu(8))) voi
 }
 
 // SPT: one persistent launch, normally launched with at most as many
-// workgroups as fit on the GPU at once (three per CU).  grid_discover finds
-// the workgroups that actually run together (normally all of them); each
-// participant solves tiles me, me + np, ..., then a grid barrier among the
-// participants, [upper levels by participant 0, a barrier,] then the fix-up
-// of the same tiles, one wave per tile.  A tile is fixed by the workgroup
-// that solved it, so its outputs and open list stay in one XCD's L2; the
+// workgroups as fit on the GPU at once (three per CU).  grid_join and
+// grid_count find the workgroups that actually run together (normally all
+// of them), per slot c = blockIdx.x % SPT_SLOTS.  Slot c's participants
+// solve the tiles of residue c (tile c + SPT_SLOTS i for class index i):
+// each first its join ticket's, then fixed rounds (participant j takes
+// i = j, j + npc, ...), and the last SPT_DYN_ROUNDS rounds' worth of tiles
+// or fewer one at a time from the slot's ticket counter, by whichever
+// participant is free (tile durations vary with the keys).  Then a grid
+// barrier among the participants, [upper levels by participant 0, a
+// barrier,] then the fix-up of the same tiles, one wave per tile: a tile is
+// fixed by the workgroup that solved it, so its outputs and open list stay
+// in one XCD's L2 (each workgroup lists the tiles it took in LDS).  The
 // summaries other workgroups read (tile mins, chunk mins, spines, counts,
 // upper keys) are written and read with device-coherent accesses, and the
 // barrier needs no fence.  The occupancy attribute guarantees
 // SPT_WAVES_PER_EU waves per SIMD.  The fix-up's LDS reuses the tile's.
 // (A cooperative launch, which has the runtime guarantee co-residency
 // instead, costs ~17 us more per launch on MI355X.)
+constexpr int SPT_DYN_ROUNDS = 3;
+// Tiles one workgroup may take from its slot's counter.  The counter hands
+// out fewer than (SPT_DYN_ROUNDS + 1) * npc tiles, so every one is taken as
+// long as SPT_DYN_CAP >= SPT_DYN_ROUNDS + 1, whatever the participant count.
+constexpr int SPT_DYN_CAP = 32;
+static_assert(SPT_DYN_CAP >= SPT_DYN_ROUNDS + 1, "the counter's tiles must all find a taker");
+
 template <bool INCL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SPT_WAVES_PER_EU)))
 void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
            TileOut to, Levels L, GridBarrier* bar) {
     __shared__ TileSmem sm;
+    __shared__ int s_dyn[SPT_DYN_CAP];  // tiles this workgroup took by ticket
+    __shared__ unsigned s_next;         // the next tile's ticket
     static_assert(NWAVE * FIX_LDS_INTS * sizeof(int) <= sizeof(TileSmem), "fix-up LDS");
-    const int2 pr = grid_discover(bar);
-    const int me = pr.x, np = pr.y;
-    if (me < 0) return;  // uniform per workgroup: not a participant
+    const int j = grid_join(bar);
+    if (j < 0) return;  // uniform per workgroup: not a participant
     // Nothing per-lane stays live across the tile loop (it would be spilled
     // to memory): the wave index is kept in a scalar, and each tile computes
     // the lane's index afresh inside a volatile asm, so the compiler cannot
     // hoist the lane-derived addresses out of the loop.
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int nt = L.nt;
-    for (int t = me; t < nt; t += np) {
+    const int c = (int)(blockIdx.x % SPT_SLOTS);
+    unsigned* const tickets = &bar->tiles[c * BAR_LINE];
+    // tiles of residue c (at least as many as slot c's workgroups: the grid
+    // has at most nt workgroups)
+    const int nc = (nt - c + SPT_SLOTS - 1) / SPT_SLOTS;
+    // schedule, known after the first tile: slot participants, fixed
+    // rounds S (the first included), first class index taken by ticket
+    int npc = 0, S = 1, tail0 = 0, ntail = 0, ndyn = 0, i = j;
+    u64 pub = 0ull;
+    while (i < nc) {
         int lane;
         asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0"
                      : "=v"(lane));
-        tile_step<INCL, true, true>(in, out, n, t, sm, to, L, wave * WAVE + lane);
+        const int tid = wave * WAVE + lane;
+        const int t = c + SPT_SLOTS * i;
+        const bool dyn = npc > 0 && i >= tail0;
+        // the next tile comes from the counter once the fixed rounds are
+        // done, while this workgroup's list has room for it; its ticket is
+        // taken inside this tile and lands in s_next before the tile's
+        // closing barrier (one slot suffices: the next write comes after the
+        // next tile's first barrier, which every wave passes only after
+        // reading this one)
+        const bool take = npc > 0 && i + npc >= tail0 && ndyn + (dyn ? 1 : 0) < SPT_DYN_CAP;
+        int v[K];
+        load_lane(in, n, t, v, tid);
+        const int pl = (tid == 0 && t > 0) ? in[t * TILE - 1] : 0;
+        solve_tile<INCL, false, true, true>(out, n, t, v, pl, sm, to, L, SegLane{0, 0u}, tid,
+                                            take ? tickets : nullptr, &s_next);
+        if (dyn) {
+            if (tid == 0) s_dyn[ndyn] = t;
+            ndyn++;
+        }
+        if (npc == 0) {  // after the first tile
+            pub = grid_count(bar, c == 0 && j == 0);
+            npc = (int)((pub >> (8 * c)) & 0xFF);
+            if (npc == 0) return;  // timed out (error word raised)
+            S = max(nc / npc - SPT_DYN_ROUNDS, 1);
+            tail0 = S * npc;     // <= nc: npc <= the slot's workgroups <= nc
+            ntail = nc - tail0;  // < (SPT_DYN_ROUNDS + 1) * npc
+            if (S > 1) {
+                i += npc;
+            } else {
+                if (tid == 0)
+                    s_next = __hip_atomic_fetch_add(tickets, 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+                __syncthreads();
+                i = tail0 + (int)min(__builtin_amdgcn_readfirstlane(s_next), (unsigned)ntail);
+            }
+        } else if (take) {
+            i = tail0 + (int)min(__builtin_amdgcn_readfirstlane(s_next), (unsigned)ntail);
+        } else {
+            i = i + npc < tail0 ? i + npc : nc;
+        }
     }
     if (nt <= 1) return;  // uniform across the grid
+    // participant index (slots in order) and count
+    int me = j, np = 0;
+#pragma unroll
+    for (int k = 0; k < SPT_SLOTS; k++) {
+        const int nk = (int)((pub >> (8 * k)) & 0xFF);
+        if (k < c) me += nk;
+        np += nk;
+    }
     const bool leader = wave == 0 && lane_id() == 0;
-    unsigned gen = grid_barrier_start(bar), j = 0;  // before this workgroup arrives
-    if (!grid_barrier_co(bar, (unsigned)np, (unsigned)me, gen, j, leader)) return;
+    unsigned gen = grid_barrier_start(bar), jb = 0;  // before this workgroup arrives
+    if (!grid_barrier_co(bar, (unsigned)np, (unsigned)me, gen, jb, leader)) return;
     if (L.upper) {
         if (me == 0) build_upper_levels<true>(L, wave);
-        if (!grid_barrier_co(bar, (unsigned)np, (unsigned)me, gen, j, leader)) return;
+        if (!grid_barrier_co(bar, (unsigned)np, (unsigned)me, gen, jb, leader)) return;
     }
     int* lds = reinterpret_cast<int*>(&sm) + wave * FIX_LDS_INTS;
-    for (int k = wave;; k += NWAVE) {
-        const long long t = (long long)me + (long long)k * np;
-        if (t >= nt) break;  // uniform per wave
-        fix_tile<INCL, false, true>(FlatMap{in, out}, n, (int)t, L, to, lds);
+    // this workgroup's tiles: S fixed rounds, then those taken by ticket
+    for (int k = wave; k < S + ndyn; k += NWAVE) {
+        const int t = k < S ? c + SPT_SLOTS * (j + k * npc)
+                            : __builtin_amdgcn_readfirstlane(s_dyn[k - S]);
+        fix_tile<INCL, false, true>(FlatMap{in, out}, n, t, L, to, lds);
     }
 }
 
@@ -357,7 +428,10 @@ static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
     Levels L;
     if (fill_levels(s, nt, L, st, cap) != APSEP_OK) return APSEP_ERR_LAUNCH;
     TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
+    // no more workgroups than tiles (each participant's first tile follows
+    // from its join ticket), and per slot no more than its count can say
     int grid = s->spt_grid < nt ? s->spt_grid : nt;
+    if (grid > SPT_SLOTS * SPT_SLOT_MAX) grid = SPT_SLOTS * SPT_SLOT_MAX;
     if (grid < 1) grid = 1;
     GridBarrier* bar = s->bar;
     hipLaunchKernelGGL(k_spt_begin, dim3(1), dim3(BAR_SUB), 0, st, bar);
