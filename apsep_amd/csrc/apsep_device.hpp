@@ -690,39 +690,68 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         smask &= vmask;
     }
 
-    // 3a. the lane's first element (always unanswered in-lane) against the
-    //     previous lane's values
-    bool near = false;  // its answer is in the previous lane
-    if (qmask & 1u) {
-        const int r = last_in_lane<INCL>(sm, tid - 1, v[0]);
-        if (r >= 0) {
-            res[0] = base - K + r;
-            qmask &= ~1u;
-            near = true;
-        }
-    }
-    // Then up to XQ more of the lane's queries, in position order.  They are
-    // prefix minima of the lane, so each is at most as large as the one
-    // before and its answer lies no later: once one has no answer in the
-    // previous lane, no later one has.  (When the first element is not a
-    // query, no element of the lane is.)  On uniform random keys this
-    // settles about two thirds of the queue's entries, at a third of a queue
-    // round's cost each.  The low 16 bits of xa mark the elements so
-    // answered; the answer of the k-th of them is at offset
-    // (xa >> (16 + 4k)) & 15 of the previous lane.  They are written with
-    // the staged or patched outputs below.
+    // 3a. The lane's first element (always unanswered in-lane; when it is
+    //     not a query, no element of the lane is) and up to XQ more of the
+    //     lane's queries, in position order, against the previous lane's
+    //     values.  The queries are prefix minima of the lane, so each is at
+    //     most as large as the one before and its answer lies no later: once
+    //     one has no answer in the previous lane, no later one has, and the
+    //     answers found are accepted up to the first miss.  All of them are
+    //     searched at once, so that their LDS reads (the query values, the
+    //     previous lane's group mins, then one group of its values per
+    //     query) overlap instead of forming a chain per query.  On uniform
+    //     random keys this settles about two thirds of the queue's entries.
+    //     The low 16 bits of xa mark the further elements so answered; the
+    //     answer of the k-th of them is at offset (xa >> (16 + 4k)) & 15 of
+    //     the previous lane.  They are written with the staged or patched
+    //     outputs below.
     unsigned xa = 0;
+    if (qmask & 1u) {
+        const int pt = tid - 1;  // >= 0: the tile's first lane has no queries
+        const int4 g = sm.gmin[pt];
+        int q0 = -1;
+        sel4<INCL, 0>(q0, g.x, g.y, g.z, g.w, v[0]);
+        // one group of the previous lane per query (group 0 when none
+        // qualifies: the read is unconditional)
+        const int4 w0 = sm.val4[max(q0, 0)][pt];
+        int j0 = -1;
+        sel4<INCL, 0>(j0, w0.x, w0.y, w0.z, w0.w, v[0]);
+        if (qmask & ~1u) {  // further queries (none on ascending keys)
+            unsigned m = qmask & ~1u;
+            int ik[XQ], xv[XQ], qk[XQ];
+            int4 wk[XQ];
 #pragma unroll
-    for (int k = 0; k < XQ; k++) {
-        if (near && qmask) {
-            const int i = __builtin_ctz(qmask);
-            const int r = last_in_lane<INCL>(sm, tid - 1, tile_value(sm, tid * K + i));
-            if (r >= 0) {
-                xa |= (1u << i) | ((unsigned)r << (16 + 4 * k));
-                qmask &= qmask - 1;
-            } else {
-                near = false;
+            for (int k = 0; k < XQ; k++) {
+                ik[k] = m ? __builtin_ctz(m) : 0;  // (position 0 stands in for none)
+                m &= m - 1;
+                xv[k] = tile_value(sm, tid * K + ik[k]);
             }
+#pragma unroll
+            for (int k = 0; k < XQ; k++) {
+                qk[k] = -1;
+                sel4<INCL, 0>(qk[k], g.x, g.y, g.z, g.w, xv[k]);
+                wk[k] = sm.val4[max(qk[k], 0)][pt];
+            }
+            if (q0 >= 0) {
+                unsigned mq = qmask & ~1u;
+                bool ok = true;
+#pragma unroll
+                for (int k = 0; k < XQ; k++) {
+                    int j = -1;
+                    sel4<INCL, 0>(j, wk[k].x, wk[k].y, wk[k].z, wk[k].w, xv[k]);
+                    if (ok && mq && qk[k] >= 0) {
+                        xa |= (1u << ik[k]) | ((unsigned)(4 * qk[k] + j) << (16 + 4 * k));
+                        mq &= mq - 1;
+                    } else {
+                        ok = false;
+                    }
+                }
+                qmask = mq | 1u;  // (bit 0 cleared below)
+            }
+        }
+        if (q0 >= 0) {  // j0 >= 0: the group's min qualifies
+            res[0] = base - K + 4 * q0 + j0;
+            qmask &= ~1u;
         }
     }
 
