@@ -35,7 +35,8 @@ EXPORTS = [
     "apsep_status_string", "apsep_version", "apsep_last_error", "apsep_scratch_create",
     "apsep_scratch_destroy", "apsep_scratch_bytes", "apsep_run",
     "apsep_launch", "apsep_run_mode", "apsep_run_noalloc", "apsep_scratch_query",
-    "apsep_scratch_check", "apsep_time_kernels", "apsep_time_kernels_rotating",
+    "apsep_scratch_check", "apsep_scratch_set_persistent_blocks", "apsep_time_kernels",
+    "apsep_time_kernels_rotating",
     "apsep_batch_plan_create", "apsep_batch_plan_destroy", "apsep_run_segmented",
     "apsep_batch_plan_use_graph", "apsep_batch_plan_tune",
     "apsep_batch_plan_set_tile_lanes", "apsep_batch_plan_tile_lanes",
@@ -106,6 +107,7 @@ def lib() -> ctypes.CDLL:
         L.apsep_run_noalloc.argtypes = [vp, vp, i64, ci, vp]
         L.apsep_scratch_query.argtypes = [vp, ctypes.POINTER(ScratchInfo)]
         L.apsep_scratch_check.argtypes = [vp, ctypes.POINTER(i32)]
+        L.apsep_scratch_set_persistent_blocks.argtypes = [vp, i32]
         L.apsep_time_kernels.argtypes = [vp, vp, vp, i64, ci, ci, vp,
                                          ctypes.POINTER(KernelTimes)]
         L.apsep_time_kernels_rotating.argtypes = [vp, vp, vp, i32, i64, ci, ci, vp,
@@ -255,6 +257,11 @@ class Scratch:
             stream = torch.cuda.current_stream(x.device)
         self.run_ptr(x.data_ptr(), out.data_ptr(), n, stream.cuda_stream, incl)
         return out
+
+    def set_persistent_blocks(self, blocks: int):
+        """SPT persistent grid of later runs (0: the default, occupancy x CUs)."""
+        _check(lib().apsep_scratch_set_persistent_blocks(self._h, int(blocks)),
+               "apsep_scratch_set_persistent_blocks")
 
     def check(self):
         """Blocking check of the device-side error word (raises on error)."""

@@ -141,6 +141,13 @@ apsep_status apsep_scratch_query(const apsep_scratch* s, apsep_scratch_info* inf
     return APSEP_OK;
 }
 
+apsep_status apsep_scratch_set_persistent_blocks(apsep_scratch* s, int32_t blocks) {
+    if (!s || blocks < 0) return APSEP_ERR_INVALID_ARG;
+    if (s->algo != APSEP_ALGO_SPT) return APSEP_ERR_UNSUPPORTED;
+    s->spt_blocks = blocks;
+    return APSEP_OK;
+}
+
 apsep_status apsep_scratch_check(apsep_scratch* s, int32_t* device_error) {
     if (!s || !device_error) return APSEP_ERR_INVALID_ARG;
     int code = 0;

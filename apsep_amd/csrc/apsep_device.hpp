@@ -1678,6 +1678,8 @@ struct GridBarrier {
     unsigned pad1[BAR_LINE - 1];
     u64 pub;          // participants per slot, one byte each (0: not yet published)
     unsigned pad2[BAR_LINE - 2];
+    unsigned orphans; // tickets of the tiles of residues no participant's slot has
+    unsigned pad3[BAR_LINE - 1];
     unsigned disc[SPT_SLOTS * BAR_LINE];   // per slot: joins | DISC_CLOSED
     unsigned tiles[SPT_SLOTS * BAR_LINE];  // per slot: tickets of dynamically taken tiles
     unsigned sub[BAR_SUB * BAR_LINE];
@@ -1701,6 +1703,7 @@ __device__ __forceinline__ void grid_barrier_reset(GridBarrier* b) {
     if (i == 0) {
         b->root = 0u;
         b->pub = 0ull;
+        b->orphans = 0u;
     }
 }
 

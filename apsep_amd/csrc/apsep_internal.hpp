@@ -27,6 +27,8 @@ struct apsep_scratch {
     int* scount;
     int2* spine;
     int spt_grid;        // SPT persistent grid size (occupancy x CUs)
+    int spt_blocks;      // SPT grid the caller asked for (0: spt_grid)
+    int* spt_link;       // SPT: per tile, the next tile of a workgroup's overflow list
     apsep::GridBarrier* bar;
     // look-back runs: control words and the published (tagged) words
     apsep::LbCtl* lb_ctl;

@@ -79,11 +79,15 @@ __global__ __launch_bounds__(FIX_WG) __attribute__((amdgpu_waves_per_eu(8))) voi
 // each first its join ticket's, then fixed rounds (participant j takes
 // i = j, j + npc, ...), and the last SPT_DYN_ROUNDS rounds' worth of tiles
 // or fewer one at a time from the slot's ticket counter, by whichever
-// participant is free (tile durations vary with the keys).  Then a grid
-// barrier among the participants, [upper levels by participant 0, a
-// barrier,] then the fix-up of the same tiles, one wave per tile: a tile is
-// fixed by the workgroup that solved it, so its outputs and open list stay
-// in one XCD's L2 (each workgroup lists the tiles it took in LDS).  The
+// participant is free (tile durations vary with the keys).  A residue whose
+// slot has no participant (a grid smaller than SPT_SLOTS, or workgroups kept
+// from starting by other work on the GPU) is solved by all participants
+// after their own, a tile per ticket.  Then a grid barrier among the
+// participants, [upper levels by participant 0, a barrier,] then the fix-up
+// of the same tiles, one wave per tile: a tile is fixed by the workgroup
+// that solved it, so its outputs and open list stay in one XCD's L2.  A
+// workgroup lists the tiles it took by ticket in LDS, the ones past
+// SPT_DYN_CAP in a linked list through `link` (one word per tile).  The
 // summaries other workgroups read (tile mins, chunk mins, spines, counts,
 // upper keys) are written and read with device-coherent accesses, and the
 // barrier needs no fence.  The occupancy attribute guarantees
@@ -91,20 +95,18 @@ __global__ __launch_bounds__(FIX_WG) __attribute__((amdgpu_waves_per_eu(8))) voi
 // (A cooperative launch, which has the runtime guarantee co-residency
 // instead, costs ~17 us more per launch on MI355X.)
 constexpr int SPT_DYN_ROUNDS = 3;
-// Tiles one workgroup may take from its slot's counter.  The counter hands
-// out fewer than (SPT_DYN_ROUNDS + 1) * npc tiles, so every one is taken as
-// long as SPT_DYN_CAP >= SPT_DYN_ROUNDS + 1, whatever the participant count.
-constexpr int SPT_DYN_CAP = 32;
-static_assert(SPT_DYN_CAP >= SPT_DYN_ROUNDS + 1, "the counter's tiles must all find a taker");
+constexpr int SPT_DYN_CAP = 32;  // tiles taken by ticket listed in LDS
 
 template <bool INCL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SPT_WAVES_PER_EU)))
 void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
-           TileOut to, Levels L, GridBarrier* bar) {
+           TileOut to, Levels L, GridBarrier* bar, int* link) {
     __shared__ TileSmem sm;
     __shared__ int s_dyn[SPT_DYN_CAP];  // tiles this workgroup took by ticket
+    __shared__ int s_head;              // and the last one listed through link
     __shared__ unsigned s_next;         // the next tile's ticket
     static_assert(NWAVE * FIX_LDS_INTS * sizeof(int) <= sizeof(TileSmem), "fix-up LDS");
+    if (threadIdx.x == 0) s_head = -1;  // (grid_join's barrier orders it)
     const int j = grid_join(bar);
     if (j < 0) return;  // uniform per workgroup: not a participant
     // Nothing per-lane stays live across the tile loop (it would be spilled
@@ -122,49 +124,88 @@ void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
     // rounds S (the first included), first class index taken by ticket
     int npc = 0, S = 1, tail0 = 0, ntail = 0, ndyn = 0, i = j;
     u64 pub = 0ull;
-    while (i < nc) {
+    // residues with tiles and no participant (known with the schedule): one
+    // ticket counter for all of them, ticket o -> the (o mod m)-th such
+    // residue's class index o / m
+    unsigned empty = 0u;
+    bool orphans = false;  // the workgroup's own residue is done
+    int t = c + SPT_SLOTS * i;
+    while (t >= 0) {
         int lane;
         asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0"
                      : "=v"(lane));
         const int tid = wave * WAVE + lane;
-        const int t = c + SPT_SLOTS * i;
-        const bool dyn = npc > 0 && i >= tail0;
-        // the next tile comes from the counter once the fixed rounds are
-        // done, while this workgroup's list has room for it; its ticket is
-        // taken inside this tile and lands in s_next before the tile's
-        // closing barrier (one slot suffices: the next write comes after the
-        // next tile's first barrier, which every wave passes only after
-        // reading this one)
-        const bool take = npc > 0 && i + npc >= tail0 && ndyn + (dyn ? 1 : 0) < SPT_DYN_CAP;
+        const bool dyn = orphans || (npc > 0 && i >= tail0);
+        // the next tile of the residue comes from the counter once the
+        // fixed rounds are done; its ticket is taken inside this tile and
+        // lands in s_next before the tile's closing barrier (one slot
+        // suffices: the next write comes after the next tile's first
+        // barrier, which every wave passes only after reading this one)
+        const bool take = !orphans && npc > 0 && i + npc >= tail0;
         int v[K];
         load_lane(in, n, t, v, tid);
         const int pl = (tid == 0 && t > 0) ? in[t * TILE - 1] : 0;
         solve_tile<INCL, false, true, true>(out, n, t, v, pl, sm, to, L, SegLane{0, 0u}, tid,
                                             take ? tickets : nullptr, &s_next);
-        if (dyn) {
-            if (tid == 0) s_dyn[ndyn] = t;
+        if (dyn) {  // a tile taken by ticket, listed for the fix-up
+            if (tid == 0) {
+                if (ndyn < SPT_DYN_CAP) {
+                    s_dyn[ndyn] = t;
+                } else {
+                    link[t] = s_head;
+                    s_head = t;
+                }
+            }
             ndyn++;
         }
-        if (npc == 0) {  // after the first tile
-            pub = grid_count(bar, c == 0 && j == 0);
-            npc = (int)((pub >> (8 * c)) & 0xFF);
-            if (npc == 0) return;  // timed out (error word raised)
-            S = max(nc / npc - SPT_DYN_ROUNDS, 1);
-            tail0 = S * npc;     // <= nc: npc <= the slot's workgroups <= nc
-            ntail = nc - tail0;  // < (SPT_DYN_ROUNDS + 1) * npc
-            if (S > 1) {
-                i += npc;
+        if (!orphans) {
+            if (npc == 0) {  // after the first tile
+                pub = grid_count(bar, c == 0 && j == 0);
+                npc = (int)((pub >> (8 * c)) & 0xFF);
+                if (npc == 0) return;  // timed out (error word raised)
+#pragma unroll
+                for (int k = 0; k < SPT_SLOTS; k++)
+                    if (k < nt && ((pub >> (8 * k)) & 0xFF) == 0) empty |= 1u << k;
+                S = max(nc / npc - SPT_DYN_ROUNDS, 1);
+                tail0 = S * npc;     // <= nc: npc <= the slot's workgroups <= nc
+                ntail = nc - tail0;  // < (SPT_DYN_ROUNDS + 1) * npc
+                if (S > 1) {
+                    i += npc;
+                } else {
+                    if (tid == 0)
+                        s_next = __hip_atomic_fetch_add(tickets, 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+                    __syncthreads();
+                    i = tail0 + (int)min(__builtin_amdgcn_readfirstlane(s_next), (unsigned)ntail);
+                }
+            } else if (take) {
+                i = tail0 + (int)min(__builtin_amdgcn_readfirstlane(s_next), (unsigned)ntail);
             } else {
+                i = i + npc < tail0 ? i + npc : nc;
+            }
+            t = i < nc ? c + SPT_SLOTS * i : -1;
+            orphans = t < 0 && empty != 0u;
+        }
+        if (orphans) {  // (only where some residue has no participant)
+            const int m = __builtin_popcount(empty);
+            const int most = (nt - __builtin_ctz(empty) + SPT_SLOTS - 1) / SPT_SLOTS;
+            for (t = -1;;) {
+                __syncthreads();  // every wave has read s_next
                 if (tid == 0)
-                    s_next = __hip_atomic_fetch_add(tickets, 1u, __ATOMIC_RELAXED,
+                    s_next = __hip_atomic_fetch_add(&bar->orphans, 1u, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
                 __syncthreads();
-                i = tail0 + (int)min(__builtin_amdgcn_readfirstlane(s_next), (unsigned)ntail);
+                const unsigned o = __builtin_amdgcn_readfirstlane(s_next);
+                const int oi = (int)(o / (unsigned)m);
+                if (oi >= most) break;  // t = -1: done
+                unsigned em = empty;
+                for (int r = (int)(o % (unsigned)m); r > 0; r--) em &= em - 1;
+                const int to_ = __builtin_ctz(em) + SPT_SLOTS * oi;
+                if (to_ < nt) {
+                    t = to_;
+                    break;
+                }
             }
-        } else if (take) {
-            i = tail0 + (int)min(__builtin_amdgcn_readfirstlane(s_next), (unsigned)ntail);
-        } else {
-            i = i + npc < tail0 ? i + npc : nc;
         }
     }
     if (nt <= 1) return;  // uniform across the grid
@@ -185,10 +226,21 @@ void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
     }
     int* lds = reinterpret_cast<int*>(&sm) + wave * FIX_LDS_INTS;
     // this workgroup's tiles: S fixed rounds, then those taken by ticket
-    for (int k = wave; k < S + ndyn; k += NWAVE) {
+    // (listed in LDS, then through link: each wave walks the list and takes
+    // every NWAVE-th, a path only a starved grid takes)
+    const int nlds = S + min(ndyn, SPT_DYN_CAP);
+    for (int k = wave; k < nlds; k += NWAVE) {
         const int t = k < S ? c + SPT_SLOTS * (j + k * npc)
                             : __builtin_amdgcn_readfirstlane(s_dyn[k - S]);
         fix_tile<INCL, false, true>(FlatMap{in, out}, n, t, L, to, lds);
+    }
+    if (ndyn > SPT_DYN_CAP) {
+        int t = __builtin_amdgcn_readfirstlane(s_head);
+        for (int k = 0; t >= 0; k++) {
+            const int nx = __builtin_amdgcn_readfirstlane(link[t]);
+            if (k % NWAVE == wave) fix_tile<INCL, false, true>(FlatMap{in, out}, n, t, L, to, lds);
+            t = nx;
+        }
     }
 }
 
@@ -260,6 +312,7 @@ constexpr int64_t common_scratch_bytes(int64_t n_cap, int tile) {
 
 int64_t single_scratch_bytes(int64_t n_cap, apsep_algo algo, int tile) {
     int64_t b = common_scratch_bytes(n_cap, tile);
+    if (algo == APSEP_ALGO_SPT) b += (round4((n_cap + tile - 1) / tile) + 4) * 4;  // link
     if (algo == APSEP_ALGO_LOOKBACK) {
         const int nt = (int)((n_cap + tile - 1) / tile);
         const int64_t ncm = chunk_slots(n_cap, tile);
@@ -295,6 +348,7 @@ static bool carve_arrays(apsep_scratch* s) {
     s->scount = (int*)carve(p, (round4(nt) + 4) * 4);
     s->spine = (int2*)carve(p, ((int64_t)nt * RS + 2) * 8);
     s->bar = (GridBarrier*)carve(p, (int64_t)sizeof(GridBarrier));
+    if (s->algo == APSEP_ALGO_SPT) s->spt_link = (int*)carve(p, (round4(nt) + 4) * 4);
     if (s->algo == APSEP_ALGO_LOOKBACK) {
         s->lb_ctl = (LbCtl*)carve(p, (int64_t)sizeof(LbCtl));
         s->lb_M = (u64*)carve(p, (round4(nt) + 4) * 8);
@@ -430,12 +484,14 @@ static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
     TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
     // no more workgroups than tiles (each participant's first tile follows
     // from its join ticket), and per slot no more than its count can say
-    int grid = s->spt_grid < nt ? s->spt_grid : nt;
+    int grid = s->spt_blocks > 0 ? s->spt_blocks : s->spt_grid;
+    if (grid > nt) grid = nt;
     if (grid > SPT_SLOTS * SPT_SLOT_MAX) grid = SPT_SLOTS * SPT_SLOT_MAX;
     if (grid < 1) grid = 1;
     GridBarrier* bar = s->bar;
     hipLaunchKernelGGL(k_spt_begin, dim3(1), dim3(BAR_SUB), 0, st, bar);
-    hipLaunchKernelGGL(k_spt<INCL>, dim3(grid), dim3(NT), 0, st, in, out, n, to, L, bar);
+    hipLaunchKernelGGL(k_spt<INCL>, dim3(grid), dim3(NT), 0, st, in, out, n, to, L, bar,
+                       s->spt_link);
     return APSEP_LAUNCH_FAILED() ? APSEP_ERR_LAUNCH : APSEP_OK;
 }
 

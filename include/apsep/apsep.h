@@ -138,6 +138,15 @@ typedef struct apsep_scratch_info {
 
 apsep_status apsep_scratch_query(const apsep_scratch* s, apsep_scratch_info* info);
 
+/* The SPT persistent grid of later runs on this scratch: blocks workgroups
+ * (0 restores the default, occupancy x CUs, persistent_blocks above).  A
+ * caller sharing the GPU with other work may ask for fewer; more than fit
+ * at once is allowed (the workgroups that start late leave).  Every tile is
+ * solved whatever the grid.  APSEP_ERR_UNSUPPORTED for a scratch of another
+ * algorithm, APSEP_ERR_INVALID_ARG for blocks < 0.  No counterpart in the
+ * reference, whose grid is fixed at num_phys (src/apsep.cuh:3024-3032). */
+apsep_status apsep_scratch_set_persistent_blocks(apsep_scratch* s, int32_t blocks);
+
 /* Blocking check of the scratch's device-side error word, then clears it.
  * *device_error != 0 means a run on this scratch was cut short on the
  * device (a persistent launch whose workgroups were not all co-resident

@@ -76,6 +76,7 @@ def test_scratch_create_without_device_reports_device_error():
 def test_run_and_launch_argument_checks():
     L = A.lib()
     assert L.apsep_run(None, 16, 32, 4, None) == A.ERR_INVALID_ARG
+    assert L.apsep_scratch_set_persistent_blocks(None, 8) == A.ERR_INVALID_ARG
     # n <= 0 is a no-op, as the reference's run*/launch* return early
     assert L.apsep_launch(1, None, None, 0, 0) == A.OK
     assert L.apsep_launch(1, None, None, -5, 0) == A.OK
