@@ -816,7 +816,12 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     // 3b. the rest through the wave's queue: compact, resolve one query per
     //     lane, hand the answers back.  In an unstaged wave the owner
     //     overwrites its placeholder (same-thread store order); entries
-    //     beyond QCAP are resolved by their owner.
+    //     beyond QCAP are resolved by their owner.  The lane that resolves
+    //     entry `lane` (the first round) keeps the entry and its answer: in
+    //     a staged wave it patches the answer into the owner's staged
+    //     outputs itself (below).
+    unsigned qitem = 0u;
+    int qans = 0;
     if (qtotal > 0) {
         unsigned m = qmask;
         int k = qoff;
@@ -829,7 +834,12 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         wave_lds_sync();
         for (int r = lane; r < min(qtotal, QCAP); r += WAVE) {
             const int item = sm.q[wave][r];
-            sm.q[wave][r] = (unsigned short)intile_answer<INCL>(sm, item >> 4, tile_value(sm, item));
+            const int a = intile_answer<INCL>(sm, item >> 4, tile_value(sm, item));
+            sm.q[wave][r] = (unsigned short)a;
+            if (!CO && (!SEG || NWAVE <= 4) && r < WAVE) {  // (RESOLVER_PATCH, below)
+                qitem = (unsigned)item;
+                qans = a;
+            }
         }
         wave_lds_sync();
         if (!staged) {
@@ -887,10 +897,11 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     // 56 -> 24 on descending input, measured, against an XOR by q << 2.)
     if (staged) {
         int* sv = reinterpret_cast<int*>(&sm.val4[0][0]);
-        if (!SEG || __ballot(sl.wallmask != 0) == 0) {
-            // no segment starts inside this wave's elements (nearly every
-            // wave of a batch): every answer is rebased to the same start
-            const int st = SEG ? sl.incoming : 0;
+        // no segment starts inside this wave's elements (nearly every wave
+        // of a batch): every answer is rebased to the same start
+        const bool one_start = !SEG || __ballot(sl.wallmask != 0) == 0;
+        const int st = SEG ? sl.incoming : 0;  // (that start, when one_start)
+        if (one_start) {
 #pragma unroll
             for (int q = 0; q < K4; q++)
                 sm.val4[q][wave * WAVE + (lane ^ (q << 1))] =
@@ -906,15 +917,35 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
                               seg_rel<SEG>(res[4 * q + 2], 4 * q + 2, base, sl),
                               seg_rel<SEG>(res[4 * q + 3], 4 * q + 3, base, sl));
         }
-        unsigned m = qmask;
-        int k = qoff;
-        while (m) {
-            const int i = __builtin_ctz(m);
-            m &= m - 1;
-            const int q = i >> 2;
-            sv[(q * NT + wave * WAVE + (lane ^ (q << 1))) * 4 + (i & 3)] =
-                seg_rel<SEG>(tile_start + (int)sm.q[wave][k], i, base, sl);
-            k++;
+        // queue answers: the first round's by the lane that resolved each
+        // (after the owners' writes above, in the same wave: LDS order),
+        // unless segment starts in the wave make the rebasing per owner;
+        // the rest by their owners
+        // (not where the two registers held for it would spill: the
+        // persistent kernel's loop and 512-lane segmented tiles)
+        constexpr bool RESOLVER_PATCH = !CO && (!SEG || NWAVE <= 4);
+        int kfrom = 0;  // owners patch entries kfrom ..
+        if (RESOLVER_PATCH && one_start) {
+            kfrom = WAVE;
+            if (lane < qtotal) {
+                const int ol = (int)(qitem >> 4) & (WAVE - 1), i = (int)(qitem & 15u), q = i >> 2;
+                const int r = tile_start + qans;
+                sv[(q * NT + wave * WAVE + (ol ^ (q << 1))) * 4 + (i & 3)] =
+                    SEG ? max(r - st, -1) : r;
+            }
+        }
+        if (qtotal > kfrom) {
+            unsigned m = qmask;
+            int k = qoff;
+            while (m) {
+                const int i = __builtin_ctz(m);
+                m &= m - 1;
+                const int q = i >> 2;
+                if (k >= kfrom)
+                    sv[(q * NT + wave * WAVE + (lane ^ (q << 1))) * 4 + (i & 3)] =
+                        seg_rel<SEG>(tile_start + (int)sm.q[wave][k], i, base, sl);
+                k++;
+            }
         }
         unsigned xm = xa & 0xFFFFu;
 #pragma unroll
