@@ -825,11 +825,19 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     if (qtotal > 0) {
         unsigned m = qmask;
         int k = qoff;
-        while (m) {
-            const int i = __builtin_ctz(m);
-            m &= m - 1;
-            if (k < QCAP) sm.q[wave][k] = (unsigned short)((tid << 4) | i);
-            k++;
+        if (qtotal <= QCAP) {  // (uniform: nearly every wave) no bound test per entry
+            unsigned short* qd = &sm.q[wave][k];
+            while (m) {
+                *qd++ = (unsigned short)((tid << 4) | __builtin_ctz(m));
+                m &= m - 1;
+            }
+        } else {
+            while (m) {
+                const int i = __builtin_ctz(m);
+                m &= m - 1;
+                if (k < QCAP) sm.q[wave][k] = (unsigned short)((tid << 4) | i);
+                k++;
+            }
         }
         wave_lds_sync();
         for (int r = lane; r < min(qtotal, QCAP); r += WAVE) {
