@@ -266,6 +266,15 @@ __device__ __forceinline__ int tile_value(const TileSmem& sm, int e) {
     return w[((e >> 2) & (K4 - 1)) * (NT * 4) + (e >> 4) * 4 + (e & 3)];
 }
 
+// The lane's K values back from LDS (four 16-byte reads).
+__device__ __forceinline__ void lane_values(const TileSmem& sm, int tid, int* w) {
+#pragma unroll
+    for (int q = 0; q < K4; q++) {
+        const int4 a = sm.val4[q][tid];
+        w[4 * q] = a.x; w[4 * q + 1] = a.y; w[4 * q + 2] = a.z; w[4 * q + 3] = a.w;
+    }
+}
+
 // DPP data-parallel-primitive controls (GFX9 encoding): a VALU operand read
 // from another lane of the same row (16 lanes) or wave, at no extra latency.
 constexpr int DPP_ROW_SHL = 0x100;      // + n: lane + n inside the row
@@ -871,28 +880,50 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
 #pragma unroll
     for (int w = 0; w < NWAVE - 1; w++)
         if (w < wave) obase += sm.wopen[w];
+    // (A lane holds a few at most, except on descending runs, where every
+    // key is open: then all sixteen, unrolled; otherwise a loop over the
+    // lane's open keys.  The values are read back from the lane's own LDS
+    // slots, which only this wave's staging below overwrites, so that the
+    // keys' registers need not stay live until here.  On random keys the
+    // unrolled form for every lane cost the tile pass ~5 us with the
+    // spine's below.)
     if (omask && obase < QS) {
+        int2* const ql = to.qlist + (size_t)t * QS;
+        if (omask == 0xFFFFu) {
+            int w[K];
+            lane_values(sm, tid, w);
 #pragma unroll
-        for (int i = 0; i < K; i++)
-            if (omask & (1u << i)) {
-                if (obase < QS)
-                    to.qlist[(size_t)t * QS + obase] = make_int2(tid * K + i, v[i]);
-                obase++;
+            for (int i = 0; i < K; i++)
+                if (obase + i < QS) ql[obase + i] = make_int2(tid * K + i, w[i]);
+        } else {
+            for (unsigned om = omask; om && obase < QS; om &= om - 1, obase++) {
+                const int i = __builtin_ctz(om);
+                ql[obase] = make_int2(tid * K + i, tile_value(sm, tid * K + i));
             }
+        }
     }
     // 5. right spine: ranks from the tile's end, the first RS recorded
     int srank = safter_w;
 #pragma unroll
     for (int w = 1; w < NWAVE; w++)
         if (w > wave) srank += sm.wspine[w];
+    // (as the open list: every key of an ascending run is on the spine)
     if (smask && srank < RS) {
+        int2* const sp = to.spine + (size_t)t * RS;
+        if (smask == 0xFFFFu) {
+            int w[K];
+            lane_values(sm, tid, w);
 #pragma unroll
-        for (int i = K - 1; i >= 0; i--)
-            if (smask & (1u << i)) {
-                if (srank < RS)
-                    st_i2<CO>(&to.spine[(size_t)t * RS + srank], make_int2(tid * K + i, v[i]));
-                srank++;
+            for (int i = K - 1; i >= 0; i--)
+                if (srank + (K - 1 - i) < RS)
+                    st_i2<CO>(&sp[srank + (K - 1 - i)], make_int2(tid * K + i, w[i]));
+        } else {
+            for (unsigned m = smask; m && srank < RS; srank++) {
+                const int i = 31 - __builtin_clz(m);
+                m &= ~(1u << i);
+                st_i2<CO>(&sp[srank], make_int2(tid * K + i, tile_value(sm, tid * K + i)));
             }
+        }
     }
     // staged outputs.  Each wave stages its 1024 outputs in its own lanes'
     // value slots: 16-byte group g of the wave (elements 4g .. 4g+3) goes to
