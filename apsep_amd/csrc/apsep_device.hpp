@@ -266,6 +266,11 @@ __device__ __forceinline__ int tile_value(const TileSmem& sm, int e) {
     return w[((e >> 2) & (K4 - 1)) * (NT * 4) + (e >> 4) * 4 + (e & 3)];
 }
 
+// Value i (0 <= i < K) of lane tid, from its own LDS slots.
+__device__ __forceinline__ int own_value(const TileSmem& sm, int tid, int i) {
+    return reinterpret_cast<const int*>(&sm.val4[(i >> 2) & (K4 - 1)][tid])[i & 3];
+}
+
 // The lane's K values back from LDS (four 16-byte reads).
 __device__ __forceinline__ void lane_values(const TileSmem& sm, int tid, int* w) {
 #pragma unroll
@@ -733,7 +738,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
             for (int k = 0; k < XQ; k++) {
                 ik[k] = m ? __builtin_ctz(m) : 0;  // (position 0 stands in for none)
                 m &= m - 1;
-                xv[k] = tile_value(sm, tid * K + ik[k]);
+                xv[k] = own_value(sm, tid, ik[k]);
             }
 #pragma unroll
             for (int k = 0; k < XQ; k++) {
@@ -898,7 +903,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         } else {
             for (unsigned om = omask; om && obase < QS; om &= om - 1, obase++) {
                 const int i = __builtin_ctz(om);
-                ql[obase] = make_int2(tid * K + i, tile_value(sm, tid * K + i));
+                ql[obase] = make_int2(tid * K + i, own_value(sm, tid, i));
             }
         }
     }
@@ -921,7 +926,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
             for (unsigned m = smask; m && srank < RS; srank++) {
                 const int i = 31 - __builtin_clz(m);
                 m &= ~(1u << i);
-                st_i2<CO>(&sp[srank], make_int2(tid * K + i, tile_value(sm, tid * K + i)));
+                st_i2<CO>(&sp[srank], make_int2(tid * K + i, own_value(sm, tid, i)));
             }
         }
     }
@@ -997,17 +1002,26 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
             }
         wave_lds_sync();
         const int wbase = tile_start + wave * (WAVE * K);
+        // the wave's outputs all inside the array (uniform; every wave but
+        // the last tile's): a scalar base and 32-bit lane offsets
+        int* const wout = out + wbase;
+        const bool whole = wbase <= n - WAVE * K;
 #pragma unroll
         for (int q = 0; q < K4; q++) {
             const int c = lane & 3, r = 16 * q + (lane >> 2);
             const int4 x = sm.val4[c][wave * WAVE + (r ^ (c << 1))];
-            const int e = wbase + q * (WAVE * 4) + lane * 4;
-            if (e + 3 < n) {
+            const unsigned o = (unsigned)(q * (WAVE * 4) + lane * 4);
+            const int e = wbase + (int)o;
+            if (whole) {
                 // streaming (non-temporal) store: the answers are not read
                 // back by this solve, and keeping them out of the caches
                 // leaves those to the keys and summaries (measured: 1.8%
                 // on the alternating-class solves, 4% on the batched
                 // configuration; non-temporal key loads were 21% slower)
+                typedef int v4i __attribute__((ext_vector_type(4)));
+                const v4i y = {x.x, x.y, x.z, x.w};
+                __builtin_nontemporal_store(y, reinterpret_cast<v4i*>(wout + o));
+            } else if (e + 3 < n) {
                 typedef int v4i __attribute__((ext_vector_type(4)));
                 const v4i y = {x.x, x.y, x.z, x.w};
                 __builtin_nontemporal_store(y, reinterpret_cast<v4i*>(out + e));
