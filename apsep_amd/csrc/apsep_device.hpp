@@ -181,6 +181,13 @@ __device__ __forceinline__ void st_u64(u64* p, u64 v) {
     if (CO) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else *p = v;
 }
+// 16 bytes stored device-coherent: the same cache policy (sc1) as an
+// agent-scope atomic store, four dwords wide (no builtin gives it).
+__device__ __forceinline__ void st_i4_dev(int* p, int4 x) {
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    const v4i y = {x.x, x.y, x.z, x.w};
+    asm volatile("global_store_dwordx4 %0, %1, off sc1" ::"v"(p), "v"(y) : "memory");
+}
 template <bool CO>
 __device__ __forceinline__ void st_i2(int2* p, int2 v) {
     if (CO) st_u64<true>(reinterpret_cast<u64*>(p), (u64)(unsigned)v.x | ((u64)(unsigned)v.y << 32));
@@ -537,7 +544,12 @@ __device__ __forceinline__ unsigned qual_bits(int p, const int* v) {
     return INCL ? (~m & 0xFFFFu) : m;
 }
 
-template <bool INCL, bool SEG = false, bool L1 = true, bool CO = false>
+// OPEN_CO: the outputs are written with device-coherent stores (through
+// this XCD's L2), so that another workgroup, on any XCD, can overwrite the
+// open elements' -1 with device-coherent stores of its own without a dirty
+// copy in this workgroup's L2 being written back over them later; the open
+// list and its count are written device-coherent too.
+template <bool INCL, bool SEG = false, bool L1 = true, bool CO = false, bool OPEN_CO = false>
 __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, const int* v,
                                            int prev_last, TileSmem& sm, const TileOut& to,
                                            const Levels& L,
@@ -803,7 +815,12 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
     // consecutive outputs now and patches the answers in afterwards.
     const bool staged = qtotal <= QCAP;
     if (!staged) {
-        if (nvalid == K) {
+        if (nvalid == K && OPEN_CO) {
+#pragma unroll
+            for (int q = 0; q < K4; q++)
+                st_i4_dev(out + base + 4 * q, make_int4(res[4 * q], res[4 * q + 1],
+                                                        res[4 * q + 2], res[4 * q + 3]));
+        } else if (nvalid == K) {
 #pragma unroll
             for (int q = 0; q < K4; q++)
                 *reinterpret_cast<int4*>(out + base + 4 * q) =
@@ -814,7 +831,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         } else {
 #pragma unroll
             for (int i = 0; i < K; i++)
-                if (i < nvalid) out[base + i] = seg_rel<SEG>(res[i], i, base, sl);
+                if (i < nvalid) st_i<OPEN_CO>(&out[base + i], seg_rel<SEG>(res[i], i, base, sl));
         }
         unsigned xm = xa & 0xFFFFu;
 #pragma unroll
@@ -899,14 +916,15 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
             lane_values(sm, tid, w);
 #pragma unroll
             for (int i = 0; i < K; i++)
-                if (obase + i < QS) ql[obase + i] = make_int2(tid * K + i, w[i]);
+                if (obase + i < QS) st_i2<OPEN_CO>(&ql[obase + i], make_int2(tid * K + i, w[i]));
         } else {
             for (unsigned om = omask; om && obase < QS; om &= om - 1, obase++) {
                 const int i = __builtin_ctz(om);
-                ql[obase] = make_int2(tid * K + i, own_value(sm, tid, i));
+                st_i2<OPEN_CO>(&ql[obase], make_int2(tid * K + i, own_value(sm, tid, i)));
             }
         }
     }
+
     // 5. right spine: ranks from the tile's end, the first RS recorded
     int srank = safter_w;
 #pragma unroll
@@ -1012,7 +1030,15 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
             const int4 x = sm.val4[c][wave * WAVE + (r ^ (c << 1))];
             const unsigned o = (unsigned)(q * (WAVE * 4) + lane * 4);
             const int e = wbase + (int)o;
-            if (whole) {
+            if (OPEN_CO) {
+                if (e < n - 3) {
+                    st_i4_dev(out + e, x);
+                } else {
+                    if (e < n) st_i<true>(&out[e], x.x);
+                    if (e < n - 1) st_i<true>(&out[e + 1], x.y);
+                    if (e < n - 2) st_i<true>(&out[e + 2], x.z);
+                }
+            } else if (whole) {
                 // streaming (non-temporal) store: the answers are not read
                 // back by this solve, and keeping them out of the caches
                 // leaves those to the keys and summaries (measured: 1.8%
@@ -1042,7 +1068,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
             stotal += sm.wspine[w];
         }
         st_i<CO>(&L.mn0[t], m);
-        st_i<CO>(&to.qcount[t], total);
+        st_i<CO || OPEN_CO>(&to.qcount[t], total);
         st_i<CO>(&to.scount[t], stotal);
         if (L1 && L.nlev > 1) atomicMin(&key_at(L, 1, t >> LOG_FAN), make_key(L.epoch, m));
         if (tk_ctr) *tk_slot = tk;
@@ -1197,15 +1223,13 @@ __device__ int find_tile_wave(const Levels& L, int tb, int v) {
 // round, an older tile one more (its spine), and only a spine longer than
 // RS falls back to chunk mins + elements.
 // ---------------------------------------------------------------------------
-// Control words of a look-back run (device memory): run tag, tile ticket
-// counter, error word; each on its own 128-byte line.
+// Control words of a look-back run (device memory): run tag and error word,
+// each on its own 128-byte line.
 struct LbCtl {
     unsigned epoch;
     unsigned pad0[31];
-    unsigned ticket;
-    unsigned pad1[31];
     int err;
-    unsigned pad2[31];
+    unsigned pad1[31];
 };
 constexpr int LB_LEVELS = MAX_LEVELS;
 

@@ -33,7 +33,7 @@ struct apsep_scratch {
     // look-back runs: control words and the published (tagged) words
     apsep::LbCtl* lb_ctl;
     unsigned long long* lb_M;
-    unsigned long long* lb_S;
+    unsigned* lb_GC;
     unsigned long long* lb_CM;
     unsigned long long* lb_G[apsep::LB_LEVELS];
 };

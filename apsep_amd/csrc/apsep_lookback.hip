@@ -1,28 +1,35 @@
 // apsep_lookback.hip -- single-pass, non-cooperative APSEP for gfx950 with
 // decoupled look-back (APSEP_ALGO_LOOKBACK).
 //
-// The reference's look-back family (src/apsep.cuh:236-254, 1720-1747) chains
-// every block behind its predecessor's final prefix.  Here each workgroup
-// takes the next tile from a ticket counter, so it only ever waits for
-// tiles whose workgroups already run, and:
-//   1. solves its tile (the shared tile solver; plain stores, read only by
-//      this workgroup);
-//   2. publishes the tile min (M), the 64 chunk mins (CM) and, after the
-//      look-back, its inclusive prefix-min (S); the last tile of each
-//      64- / 4096- / 262144-tile group publishes the group min (G1..G3);
-//   3. looks back over its predecessors' S and M words, 64 per wave load,
-//      for its exclusive prefix-min P;
-//   4. answers its open elements that P beats: the window of the 64
-//      previous tile mins, the group mins for older tiles, then the found
-//      tile's chunk mins and one 64-key chunk of the input.
+// The reference's look-back family (src/apsep.cuh:236-254, 1720-1747) has
+// each block solve its tile, publish its summary and look back over its
+// predecessors' published summaries for the answers its tile leaves open.
+// Here one launch holds two kinds of workgroup:
+//   tile workgroups (one per tile, index = tile) solve the tile with the
+//     shared tile solver, publish its 128 chunk mins (CM) and its min (M),
+//     and the last tile of a 64-tile group to publish also publishes the
+//     group's min (G1), and so on up the levels (G2, G3: group counters
+//     tell which is last).  A tile workgroup never waits for another.
+//   answer workgroups (after the tiles, one wave per tile) look back: the
+//     exclusive prefix-min P of tile t is the min of the published entries
+//     before t in each enclosing group (at most 63 per level, loaded at
+//     once); then the open elements that P beats are answered from the 64
+//     previous tile mins, the group mins for older tiles, the found tile's
+//     chunk mins and one 64-key chunk of the input, and written over the
+//     tile's -1.
+// Keeping the look-back out of the tile workgroups keeps its round trips
+// from holding a tile's LDS (three tiles per CU): the tiles run as in the
+// multi-launch tile pass.
 // Nothing crosses workgroups through a fence: every published word is a
 // 64-bit (run tag << 32 | value) written with one agent-scope atomic store
 // and read with agent-scope atomic loads, so a word is either this run's
-// complete value or recognisably stale.  A reader that meets a stale word
-// polls it (bounded: past the limit the run is flagged as failed and every
-// wait ends).  The run tag lives in device memory and is advanced by a
-// one-thread kernel before the tile kernel, so the launch sequence holds no
-// host-baked state and can be captured into a graph.
+// complete value or recognisably stale; the open list and the outputs the
+// answers overwrite are written device-coherent, all before the tile's M.
+// A reader that meets a stale word polls it (bounded: past the limit the
+// run is flagged as failed and every wait ends).  The run tag lives in
+// device memory and is advanced by a one-workgroup kernel before the solve,
+// so the launch sequence holds no host-baked state and can be captured
+// into a graph.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -48,8 +55,9 @@ __device__ __forceinline__ u64 tagged(unsigned tag, int v) {
 struct LbView {
     LbCtl* ctl;
     u64* M;                   // [nt] tile mins
-    u64* S;                   // [nt] inclusive prefix-mins
     u64* G[LB_LEVELS];        // G[0] unused; G[l] = min of 64^l tiles
+    unsigned* GC;             // group counters: entries of group g at level l - 1
+                              // published, at GC[gn[1] + .. + gn[l - 1] + g]
     int gn[LB_LEVELS];        // entries per level (gn[0] = nt)
     u64* CM;                  // [nt * CPT] chunk mins
     unsigned tag;
@@ -82,6 +90,10 @@ __device__ __forceinline__ const u64* lb_level_ptr(const LbView& V, int lvl) {
 __device__ __forceinline__ int lb_count(const LbView& V, int lvl) {
     return lvl == 0 ? V.gn[0] : lvl == 1 ? V.gn[1] : lvl == 2 ? V.gn[2] : lvl == 3 ? V.gn[3]
                                                                         : V.gn[4];
+}
+__device__ __forceinline__ int lb_gc_base(const LbView& V, int lvl) {
+    static_assert(LB_LEVELS == 5, "levels");
+    return (lvl > 1 ? V.gn[1] : 0) + (lvl > 2 ? V.gn[2] : 0) + (lvl > 3 ? V.gn[3] : 0);
 }
 __device__ __forceinline__ int lb_level(const LbView& V, int lvl, int idx) {
     return lb_wait(V, lb_level_ptr(V, lvl) + idx);
@@ -151,86 +163,92 @@ __device__ __forceinline__ void lb_publish_group(const LbView& V, int lvl, int g
     if (lane == 0) st_agent(const_cast<u64*>(lb_level_ptr(V, lvl)) + g, tagged(V.tag, m));
 }
 
+// Tile role: solve tile t (its open elements' -1 outputs written
+// device-coherent: an answer role may overwrite them), then publish its
+// chunk mins and, once every summary of the tile has been written, its min.
 template <bool INCL>
-__global__ __launch_bounds__(NT) void k_lookback(const int* __restrict__ in,
-                                                 int* __restrict__ out, int n, TileOut to,
-                                                 Levels L, LbView V) {
-    __shared__ TileSmem sm;
-    __shared__ int s_t;
-    __shared__ int lds[WAVE];
-    if (threadIdx.x == 0)
-        s_t = (int)__hip_atomic_fetch_add(&V.ctl->ticket, 1u, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int t = s_t;
-    if (t >= V.nt) return;
-    V.tag = V.ctl->epoch;  // set by the preceding one-thread kernel
+__device__ __forceinline__ void lb_solve(const int* __restrict__ in, int* __restrict__ out,
+                                         int n, int t, TileSmem& sm, const TileOut& to,
+                                         const Levels& L, const LbView& V) {
+    const int tid = (int)threadIdx.x;
     int v[K];
     load_lane(in, n, t, v);
-    const int pl = (threadIdx.x == 0 && t > 0) ? in[t * TILE - 1] : 0;
-    solve_tile<INCL>(out, n, t, v, pl, sm, to, L);  // ends with a workgroup barrier
-    if (threadIdx.x >= WAVE) return;
-    const int lane = threadIdx.x;
-    const int tile_start = t * TILE;
-
-    // 2. publish the chunk mins and the tile min
+    const int pl = (tid == 0 && t > 0) ? in[t * TILE - 1] : 0;
+    solve_tile<INCL, false, false, false, true>(out, n, t, v, pl, sm, to, L);  // ends with a barrier
+    if (tid < CPT)
+        st_agent(&V.CM[(size_t)t * CPT + tid], tagged(V.tag, to.cmin[(size_t)t * CPT + tid]));
+    wait_own_memory_ops();  // open list and count, open outputs, chunk mins: complete
+    __syncthreads();
+    if (tid >= WAVE) return;
     int tmin = sm.wmin[0];
 #pragma unroll
     for (int w = 1; w < NWAVE; w++) tmin = min(tmin, sm.wmin[w]);
-    for (int c = lane; c < CPT; c += WAVE)
-        st_agent(&V.CM[(size_t)t * CPT + c], tagged(V.tag, to.cmin[(size_t)t * CPT + c]));
-    if (lane == 0) st_agent(&V.M[t], tagged(V.tag, tmin));
+    if (tid == 0) {
+        st_agent(&V.M[t], tagged(V.tag, tmin));
+        wait_own_memory_ops();
+    }
+    // the last entry of a group to be published publishes the group's min
+    // (every other entry of the group was published before its count)
+    int idx = t;
+    for (int l = 1; l < LB_LEVELS && lb_count(V, l) > 0; l++) {
+        const int g = idx >> LOG_FAN;
+        const int members = min(1 << LOG_FAN, lb_count(V, l - 1) - (g << LOG_FAN));
+        unsigned c = 0u;
+        if (tid == 0)
+            c = __hip_atomic_fetch_add(&V.GC[lb_gc_base(V, l) + g], 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        if ((int)__builtin_amdgcn_readfirstlane(c) + 1 != members) break;
+        lb_publish_group(V, l, g);
+        if (tid == 0) wait_own_memory_ops();
+        idx = g;
+    }
+}
 
-    // 3. look back: exclusive prefix-min over tiles < t
+// Answer role, one wave: the open elements of tile t that have an answer
+// before the tile, written device-coherent over the tile's -1.
+template <bool INCL>
+__device__ void lb_answer(const int* __restrict__ in, int* __restrict__ out, int n, int t,
+                          const TileOut& to, const LbView& V, int* lds) {
+    const int lane = (int)threadIdx.x & (WAVE - 1);
+    const int tile_start = t * TILE;
+    lb_wait(V, &V.M[t]);  // the tile is solved
+    // (a failed wait leaves the tile's summaries unknown: nothing is written)
+    if ((unsigned)(ld_agent(&V.M[t]) >> 32) != V.tag) return;
+
+    // look back: the exclusive prefix-min P over tiles < t, from the
+    // published mins of the tiles before t in its 64-tile group and of the
+    // earlier groups in each enclosing group (at most 63 entries per level,
+    // all loaded at once)
     int P = INF;
-    for (int j = t - 1; j >= 0;) {
-        const int idx = j - lane;
-        u64 ws = 0, wm = 0;
-        if (idx >= 0) {
-            ws = ld_agent(&V.S[idx]);
-            wm = ld_agent(&V.M[idx]);
-        }
-        const bool vs = idx >= 0 && (unsigned)(ws >> 32) == V.tag;
-        const bool vm = idx >= 0 && (unsigned)(wm >> 32) == V.tag;
-        const u64 bs = __ballot(vs);
-        const int ks = bs ? (int)__builtin_ctzll(bs) : WAVE;  // nearest inclusive prefix
-        const u64 need = __ballot(lane < ks && idx >= 0);
-        if (need & ~__ballot(vm)) {
-            // a predecessor has not published its tile min yet: poll (bounded)
-            if (lb_wait(V, &V.M[max(j - (int)__builtin_ctzll(need & ~__ballot(vm)), 0)]) == INF &&
-                __hip_atomic_load(&V.ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                break;
-            continue;
+    {
+        u64 w[LB_LEVELS];
+        const u64* p[LB_LEVELS];
+        int idx = t;
+#pragma unroll
+        for (int l = 0; l < LB_LEVELS; l++) {
+            const int j = ((idx >> LOG_FAN) << LOG_FAN) + lane;
+            const bool use = j < idx && (l == 0 || lb_count(V, l) > 0);
+            p[l] = use ? lb_level_ptr(V, l) + j : nullptr;
+            w[l] = use ? ld_agent(p[l]) : tagged(V.tag, INF);
+            idx >>= LOG_FAN;
         }
         int x = INF;
-        if (lane < ks && idx >= 0) x = (int)(unsigned)wm;
-        if (lane == ks) x = (int)(unsigned)ws;
-        P = min(P, wave_min(x));
-        if (ks < WAVE) break;
-        j -= WAVE;
-    }
-    if (lane == 0) st_agent(&V.S[t], tagged(V.tag, min(P, tmin)));
-
-    // group mins, by the last tile of each group
-    for (int l = 1; l < LB_LEVELS && lb_count(V, l) > 0; l++) {
-        const int span_log = LOG_FAN * l;
-        const bool last = ((t + 1) & ((1 << span_log) - 1)) == 0 || t == V.nt - 1;
-        if (!last) break;
-        lb_publish_group(V, l, t >> span_log);
+#pragma unroll
+        for (int l = 0; l < LB_LEVELS; l++)
+            x = min(x, (unsigned)(w[l] >> 32) == V.tag ? (int)(unsigned)w[l] : lb_wait(V, p[l]));
+        P = wave_min(x);
     }
 
-    // 4. open elements that P beats (same rules as the multi-launch fix-up)
-    if (t == 0) return;
-    const int v1 = __shfl(v[0], 0, WAVE);  // the tile's first element (lane 0's first value)
-    if (!qual<INCL>(P, v1)) return;
-    const int cnt = to.qcount[t];
-    const int tw = t - WAVE + lane;
-    lds[lane] = tw >= 0 ? lb_wait(V, &V.M[tw]) : INF;
-    wave_lds_sync();
+    // open elements that P beats (same rules as the multi-launch fix-up)
+    if (t == 0 || !qual<INCL>(P, in[tile_start])) return;  // the first element bounds them all
+    const int cnt = ld_i<true>(&to.qcount[t]);
     if (cnt <= QS) {
         int2 e = make_int2(0, INF);
-        if (lane < cnt) e = to.qlist[(size_t)t * QS + lane];
-        const bool active = lane < cnt && qual<INCL>(P, e.y);
+        if (lane < cnt) e = ld_i2<true>(&to.qlist[(size_t)t * QS + lane]);
+        const int tw = t - WAVE + lane;
+        lds[lane] = tw >= 0 ? lb_wait(V, &V.M[tw]) : INF;
+        wave_lds_sync();
+        const bool active = lane < cnt && qual<INCL>(P, e.y) && e.x >= 0 && e.x < n - tile_start;
         const int val = e.y;
         int ts = -1;
         if (active) {
@@ -246,7 +264,7 @@ __global__ __launch_bounds__(NT) void k_lookback(const int* __restrict__ in,
             const int r = lb_find_tile<INCL>(V, t - WAVE, __shfl(val, l, WAVE));
             if (lane == l) ts = r;
         }
-        if (active) out[tile_start + e.x] = ts < 0 ? -1 : lb_answer_in_tile<INCL>(V, in, ts, val);
+        if (active && ts >= 0) st_i<true>(&out[tile_start + e.x], lb_answer_in_tile<INCL>(V, in, ts, val));
         return;
     }
     // More open elements than recorded: rescan the tile (its prefix minima
@@ -279,7 +297,7 @@ __global__ __launch_bounds__(NT) void k_lookback(const int* __restrict__ in,
         const bool open = valid && !(ex_real && qual<INCL>(ex, xv));
         bool todo = open && qual<INCL>(P, xv);
         if (todo && c_ok && qual<INCL>(c_val, xv)) {
-            out[idx] = c_pos;
+            st_i<true>(&out[idx], c_pos);
             todo = false;
         }
         u64 m = __ballot(todo);
@@ -296,12 +314,12 @@ __global__ __launch_bounds__(NT) void k_lookback(const int* __restrict__ in,
             c_val = readlane(av, l);
             c_ok = c_pos >= 0;
             if (lane == l) {
-                out[idx] = c_pos;
+                st_i<true>(&out[idx], c_pos);
                 todo = false;
             }
             // the keys after lane l that this answer qualifies for
             if (todo && lane > l && c_ok && qual<INCL>(c_val, xv)) {
-                out[idx] = c_pos;
+                st_i<true>(&out[idx], c_pos);
                 todo = false;
             }
             m = __ballot(todo);
@@ -312,14 +330,40 @@ __global__ __launch_bounds__(NT) void k_lookback(const int* __restrict__ in,
     }
 }
 
-// next run tag (never 0, which fresh memory holds), ticket counter and error
-// word reset
-__global__ void k_lookback_begin(LbCtl* c) {
-    unsigned e = c->epoch + 1u;
-    if (e == 0u) e = 1u;
-    c->epoch = e;
-    c->ticket = 0u;
-    c->err = 0;
+// Workgroups 0 .. nt - 1 solve tile blockIdx.x; the nfix after them answer
+// the open elements of blocks of NWAVE tiles (block j, j + nfix, ..., one
+// wave per tile).  Only answer workgroups wait, and only for words that tile
+// workgroups publish; nfix stays below the number of workgroups that fit on
+// the GPU at once, so tile workgroups always find room, whatever order the
+// workgroups start in.  (Started in index order, as the hardware does, the
+// answer workgroups come after every tile.)
+template <bool INCL>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_lookback(
+    const int* __restrict__ in, int* __restrict__ out, int n, TileOut to, Levels L, LbView V,
+    int nfix) {
+    __shared__ TileSmem sm;
+    V.tag = V.ctl->epoch;  // set by the preceding kernel
+    const int b = (int)blockIdx.x;
+    if (b < V.nt) {
+        lb_solve<INCL>(in, out, n, b, sm, to, L, V);
+        return;
+    }
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    int* lds = reinterpret_cast<int*>(&sm) + wave * WAVE;
+    for (int t = (b - V.nt) * NWAVE + wave; t < V.nt; t += nfix * NWAVE)
+        lb_answer<INCL>(in, out, n, t, to, V, lds);
+}
+
+// next run tag (never 0, which fresh memory holds), error word and group
+// counters reset (one workgroup)
+__global__ __launch_bounds__(256) void k_lookback_begin(LbCtl* c, unsigned* gc, int ngc) {
+    if (threadIdx.x == 0) {
+        unsigned e = c->epoch + 1u;
+        if (e == 0u) e = 1u;
+        c->epoch = e;
+        c->err = 0;
+    }
+    for (int i = (int)threadIdx.x; i < ngc; i += (int)blockDim.x) gc[i] = 0u;
 }
 
 template <bool INCL>
@@ -335,7 +379,6 @@ static apsep_status run_lookback_t(apsep_scratch* s, const int* in, int* out, in
     LbView V{};
     V.ctl = s->lb_ctl;
     V.M = s->lb_M;
-    V.S = s->lb_S;
     V.CM = s->lb_CM;
     V.nt = nt;
     int sz[MAX_LEVELS];
@@ -345,8 +388,16 @@ static apsep_status run_lookback_t(apsep_scratch* s, const int* in, int* out, in
         V.G[l] = s->lb_G[l];
         V.gn[l] = l < nl ? sz[l] : 0;
     }
-    hipLaunchKernelGGL(k_lookback_begin, dim3(1), dim3(1), 0, st, s->lb_ctl);
-    hipLaunchKernelGGL(k_lookback<INCL>, dim3(nt), dim3(NT), 0, st, in, out, n, to, L, V);
+    V.GC = s->lb_GC;
+    int ngc = 0;  // group counters, level after level
+    for (int l = 1; l < LB_LEVELS; l++) ngc += V.gn[l];
+    hipLaunchKernelGGL(k_lookback_begin, dim3(1), dim3(256), 0, st, s->lb_ctl, s->lb_GC, ngc);
+    // answer workgroups: one per block of NWAVE tiles, at most two per CU
+    // (the kernel fits three per CU)
+    const int nb = (nt + NWAVE - 1) / NWAVE;
+    const int nfix = max(1, min(nb, 2 * s->num_cus));
+    hipLaunchKernelGGL(k_lookback<INCL>, dim3(nt + nfix), dim3(NT), 0, st, in, out, n, to, L, V,
+                       nfix);
     return APSEP_LAUNCH_FAILED() ? APSEP_ERR_LAUNCH : APSEP_OK;
 }
 

@@ -318,7 +318,8 @@ int64_t single_scratch_bytes(int64_t n_cap, apsep_algo algo, int tile) {
         const int64_t ncm = chunk_slots(n_cap, tile);
         const LevelSizes ls = level_sizes_of(nt < 1 ? 1 : nt);
         b += (int64_t)sizeof(LbCtl);
-        b += 2 * (round4(nt) + 4) * 8;                // tile mins, inclusive prefixes
+        b += (round4(nt) + 4) * 8;                    // tile mins
+        b += (round4(nt) + 4) * 4;                    // group counters (fewer than nt)
         b += (round4(ncm) + 4) * 8;                   // chunk mins
         for (int l = 1; l < ls.nl; l++) b += (round4(ls.sz[l]) + 4) * 8;  // group mins
         b += 256 * (4 + LB_LEVELS);
@@ -352,7 +353,7 @@ static bool carve_arrays(apsep_scratch* s) {
     if (s->algo == APSEP_ALGO_LOOKBACK) {
         s->lb_ctl = (LbCtl*)carve(p, (int64_t)sizeof(LbCtl));
         s->lb_M = (u64*)carve(p, (round4(nt) + 4) * 8);
-        s->lb_S = (u64*)carve(p, (round4(nt) + 4) * 8);
+        s->lb_GC = (unsigned*)carve(p, (round4(nt) + 4) * 4);
         s->lb_CM = (u64*)carve(p, (round4(ncm) + 4) * 8);
         for (int l = 1; l < nl; l++) s->lb_G[l] = (u64*)carve(p, (round4(sz[l]) + 4) * 8);
     }
