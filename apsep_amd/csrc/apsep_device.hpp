@@ -939,12 +939,12 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
 #pragma unroll
             for (int i = K - 1; i >= 0; i--)
                 if (srank + (K - 1 - i) < RS)
-                    st_i2<CO>(&sp[srank + (K - 1 - i)], make_int2(tid * K + i, w[i]));
+                    st_i2<CO || OPEN_CO>(&sp[srank + (K - 1 - i)], make_int2(tid * K + i, w[i]));
         } else {
             for (unsigned m = smask; m && srank < RS; srank++) {
                 const int i = 31 - __builtin_clz(m);
                 m &= ~(1u << i);
-                st_i2<CO>(&sp[srank], make_int2(tid * K + i, own_value(sm, tid, i)));
+                st_i2<CO || OPEN_CO>(&sp[srank], make_int2(tid * K + i, own_value(sm, tid, i)));
             }
         }
     }
@@ -1069,7 +1069,7 @@ __device__ __forceinline__ void solve_tile(int* __restrict__ out, int n, int t, 
         }
         st_i<CO>(&L.mn0[t], m);
         st_i<CO || OPEN_CO>(&to.qcount[t], total);
-        st_i<CO>(&to.scount[t], stotal);
+        st_i<CO || OPEN_CO>(&to.scount[t], stotal);
         if (L1 && L.nlev > 1) atomicMin(&key_at(L, 1, t >> LOG_FAN), make_key(L.epoch, m));
         if (tk_ctr) *tk_slot = tk;
     }

@@ -129,12 +129,31 @@ __device__ int lb_find_tile(const LbView& V, int tb, int v) {
     return found;
 }
 
-// Rightmost element < tile s's end qualifying for v, given that s's min
-// qualifies (per lane): chunk mins from the published words, then the chunk.
+// Rightmost element of tile s qualifying for v, given that s's min
+// qualifies (per lane): the tile's right spine (its strict suffix minima,
+// lowest rank = nearest) with its count, 16 ranks per round; past the
+// recorded ranks the published chunk mins, then the chunk.
 template <bool INCL>
-__device__ __forceinline__ int lb_answer_in_tile(const LbView& V, const int* __restrict__ in,
-                                                 int s, int v) {
-    // 16 words per round, loaded together; a stale word is polled alone
+__device__ int lb_answer_in_tile(const LbView& V, const int* __restrict__ in,
+                                 const TileOut& to, int s, int v) {
+    const int4* sp = reinterpret_cast<const int4*>(to.spine + (size_t)s * RS);
+    const int cnt = ld_i<true>(&to.scount[s]);  // (loaded with the first ranks)
+    int pos = -1;
+#pragma unroll
+    for (int h = 0; h < RS / 16; h++) {
+        if (h > 0 && (pos >= 0 || cnt <= 16 * h)) break;
+        int4 w[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) w[k] = ld_i4<true>(&sp[8 * h + k]);
+#pragma unroll
+        for (int k = 7; k >= 0; k--) {
+            const int r = 16 * h + 2 * k;
+            if (r + 1 < cnt && qual<INCL>(w[k].w, v)) pos = w[k].z;
+            if (r < cnt && qual<INCL>(w[k].y, v)) pos = w[k].x;
+        }
+    }
+    if (pos >= 0) return s * TILE + pos;
+    // 16 chunk mins per round; a stale word is polled alone
     int c = -1;
     const u64* cm = V.CM + (size_t)s * CPT;
     for (int b = CPT - 16; b >= 0 && c < 0; b -= 16) {
@@ -211,18 +230,15 @@ __device__ void lb_answer(const int* __restrict__ in, int* __restrict__ out, int
                           const TileOut& to, const LbView& V, int* lds) {
     const int lane = (int)threadIdx.x & (WAVE - 1);
     const int tile_start = t * TILE;
-    lb_wait(V, &V.M[t]);  // the tile is solved
-    // (a failed wait leaves the tile's summaries unknown: nothing is written)
-    if ((unsigned)(ld_agent(&V.M[t]) >> 32) != V.tag) return;
-
-    // look back: the exclusive prefix-min P over tiles < t, from the
-    // published mins of the tiles before t in its 64-tile group and of the
-    // earlier groups in each enclosing group (at most 63 entries per level,
-    // all loaded at once)
-    int P = INF;
+    // Loads in as few dependent rounds as the data allows.  Round 1: the
+    // tile's own min (is it solved?), the entries for P (the published mins
+    // of the tiles before t in its 64-tile group and of the earlier groups in
+    // each enclosing group: at most 63 per level), the 64 previous tile mins
+    // (the search window) and the tile's first key.
+    const u64 wt = ld_agent(&V.M[t]);
+    u64 w[LB_LEVELS];
+    const u64* p[LB_LEVELS];
     {
-        u64 w[LB_LEVELS];
-        const u64* p[LB_LEVELS];
         int idx = t;
 #pragma unroll
         for (int l = 0; l < LB_LEVELS; l++) {
@@ -232,21 +248,29 @@ __device__ void lb_answer(const int* __restrict__ in, int* __restrict__ out, int
             w[l] = use ? ld_agent(p[l]) : tagged(V.tag, INF);
             idx >>= LOG_FAN;
         }
-        int x = INF;
-#pragma unroll
-        for (int l = 0; l < LB_LEVELS; l++)
-            x = min(x, (unsigned)(w[l] >> 32) == V.tag ? (int)(unsigned)w[l] : lb_wait(V, p[l]));
-        P = wave_min(x);
     }
+    const int tw = t - WAVE + lane;
+    const u64 ww = tw >= 0 ? ld_agent(&V.M[tw]) : tagged(V.tag, INF);
+    const int v1 = in[tile_start];
+    if ((unsigned)(wt >> 32) != V.tag) {
+        lb_wait(V, &V.M[t]);
+        // (a failed wait leaves the tile's summaries unknown: nothing is written)
+        if ((unsigned)(ld_agent(&V.M[t]) >> 32) != V.tag) return;
+    }
+    // Round 2 (written before the tile's min): its open count and list.
+    const int cnt = ld_i<true>(&to.qcount[t]);
+    int2 e = make_int2(0, INF);
+    if (lane < QS) e = ld_i2<true>(&to.qlist[(size_t)t * QS + lane]);
+    int x = INF;
+#pragma unroll
+    for (int l = 0; l < LB_LEVELS; l++)
+        x = min(x, (unsigned)(w[l] >> 32) == V.tag ? (int)(unsigned)w[l] : lb_wait(V, p[l]));
+    const int P = wave_min(x);  // exclusive prefix-min over tiles < t
 
     // open elements that P beats (same rules as the multi-launch fix-up)
-    if (t == 0 || !qual<INCL>(P, in[tile_start])) return;  // the first element bounds them all
-    const int cnt = ld_i<true>(&to.qcount[t]);
+    if (t == 0 || !qual<INCL>(P, v1)) return;  // the first element bounds them all
     if (cnt <= QS) {
-        int2 e = make_int2(0, INF);
-        if (lane < cnt) e = ld_i2<true>(&to.qlist[(size_t)t * QS + lane]);
-        const int tw = t - WAVE + lane;
-        lds[lane] = tw >= 0 ? lb_wait(V, &V.M[tw]) : INF;
+        lds[lane] = (unsigned)(ww >> 32) == V.tag ? (int)(unsigned)ww : lb_wait(V, &V.M[tw]);
         wave_lds_sync();
         const bool active = lane < cnt && qual<INCL>(P, e.y) && e.x >= 0 && e.x < n - tile_start;
         const int val = e.y;
@@ -264,7 +288,8 @@ __device__ void lb_answer(const int* __restrict__ in, int* __restrict__ out, int
             const int r = lb_find_tile<INCL>(V, t - WAVE, __shfl(val, l, WAVE));
             if (lane == l) ts = r;
         }
-        if (active && ts >= 0) st_i<true>(&out[tile_start + e.x], lb_answer_in_tile<INCL>(V, in, ts, val));
+        if (active && ts >= 0)
+            st_i<true>(&out[tile_start + e.x], lb_answer_in_tile<INCL>(V, in, to, ts, val));
         return;
     }
     // More open elements than recorded: rescan the tile (its prefix minima
@@ -307,7 +332,7 @@ __device__ void lb_answer(const int* __restrict__ in, int* __restrict__ out, int
             const int r = lb_find_tile<INCL>(V, t, v);
             int a = -1, av = 0;
             if (lane == l && r >= 0) {
-                a = lb_answer_in_tile<INCL>(V, in, r, v);
+                a = lb_answer_in_tile<INCL>(V, in, to, r, v);
                 if (a >= 0) av = in[a];
             }
             c_pos = readlane(a, l);
