@@ -688,11 +688,36 @@ def bench_structured(args, torch, apsep_amd):
 
 def bench_sweep(args, torch, apsep_amd):
     """Useful GB/s of the plain solve for N = 2^14 .. 2^27 on the three
-    classes, cache-warm (repeated solves of the same array) and with the
-    256 MB last-level cache flushed before every solve (a 1 GiB buffer is
-    rewritten between timed solves; only the solve is inside the events)."""
+    classes in three cache regimes, beside a plain device copy of the same
+    N in the same regimes (median of --steps timings, only the solve or the
+    copy inside the events):
+      warm           repeated solves of the same array;
+      flushed        a 1 GiB buffer rewritten before every solve, which
+                     evicts the 256 MB last-level cache and leaves it full
+                     of dirty lines to be written back during the solve;
+      flushed_read   a 1 GiB buffer read (summed) before every solve, which
+                     evicts the cache and leaves it clean."""
     stream = torch.cuda.current_stream()
     flush = torch.empty(1 << 28, dtype=torch.int32, device="cuda")
+    flush.fill_(1)
+    acc = torch.empty((), dtype=torch.int64, device="cuda")
+
+    def timed(fn, regime):
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        for i in range(args.steps):
+            if regime == "flushed":
+                flush.fill_(i)
+            elif regime == "flushed_read":
+                torch.sum(flush, dim=0, dtype=torch.int64, out=acc)
+            evs[i][0].record(stream)
+            fn()
+            evs[i][1].record(stream)
+        torch.cuda.synchronize()
+        return sorted(a.elapsed_time(b) for a, b in evs)[args.steps // 2]
+
+    regimes = ("warm", "flushed", "flushed_read")
     rows = []
     with apsep_amd.Scratch(args.algo, 1 << 27) as s:
         for lg in range(14, 28):
@@ -703,25 +728,17 @@ def bench_sweep(args, torch, apsep_amd):
             for k in CLASSES:
                 for _ in range(3):
                     s.run(xs[k], out, stream=stream)
-                evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-                       for _ in range(2 * args.steps)]
-                torch.cuda.synchronize()
-                for i in range(args.steps):  # warm
-                    evs[i][0].record(stream)
-                    s.run(xs[k], out, stream=stream)
-                    evs[i][1].record(stream)
-                for i in range(args.steps, 2 * args.steps):  # flushed
-                    flush.fill_(i)
-                    evs[i][0].record(stream)
-                    s.run(xs[k], out, stream=stream)
-                    evs[i][1].record(stream)
-                torch.cuda.synchronize()
-                warm = sorted(a.elapsed_time(b) for a, b in evs[:args.steps])[args.steps // 2]
-                cold = sorted(a.elapsed_time(b) for a, b in evs[args.steps:])[args.steps // 2]
-                row[k] = {"warm_GBps": round(8.0 * n / (warm * 1e-3) / 1e9, 1),
-                          "flushed_GBps": round(8.0 * n / (cold * 1e-3) / 1e9, 1),
-                          "warm_us": round(warm * 1e3, 2), "flushed_us": round(cold * 1e3, 2)}
+                row[k] = {}
+                for reg in regimes:
+                    ms = timed(lambda: s.run(xs[k], out, stream=stream), reg)
+                    row[k][reg + "_GBps"] = round(8.0 * n / (ms * 1e-3) / 1e9, 1)
+                    row[k][reg + "_us"] = round(ms * 1e3, 2)
             s.check()
+            src = xs["random"]
+            row["copy"] = {}
+            for reg in regimes:
+                ms = timed(lambda: out.copy_(src), reg)
+                row["copy"][reg + "_GBps"] = round(8.0 * n / (ms * 1e-3) / 1e9, 1)
             rows.append(row)
             print("sweep n=2^%d %s" % (lg, json.dumps(row)), file=sys.stderr, flush=True)
     best = max(r["random"]["warm_GBps"] for r in rows)
