@@ -791,6 +791,7 @@ def run_single(args, torch, apsep_amd):
         per, per_warm, total_ms, rot, warm = time_solves(torch, apsep_amd, s, xs, outs,
                                                          args.steps, args.warmup, stream,
                                                          per_reps)
+        cold = cold_clean(torch, s, xs, outs, n, stream)
         algo_name = {v: k for k, v in apsep_amd.ALGOS.items()}[s.algo]
     useful = 8.0 * n
     per_class = {k: round(useful / (per[k] * 1e-3) / 1e9, 1) for k in CLASSES}
@@ -848,6 +849,7 @@ def run_single(args, torch, apsep_amd):
         "kernel_us": _kernel_us(rot),
         "kernel_us_warm": dict(_kernel_us_from(warm), per_class={
             k: _kernel_us(t) for k, t in warm.items()}),
+        "cold_clean": cold,
         "copy_gbs": round(copy_gbs(torch, n, stream, rotate=3), 1),
         "copy_gbs_warm": round(copy_gbs(torch, n, stream, rotate=1), 1),
     }
@@ -860,6 +862,41 @@ def run_single(args, torch, apsep_amd):
     del xs, outs
     torch.cuda.empty_cache()
     return rec
+
+
+def cold_clean(torch, s, xs, outs, n, stream, reps=10):
+    """Useful GB/s per class with a clean cold cache: before each timed
+    solve a 1 GiB buffer is read (summed), which evicts the 256 MB
+    last-level cache without leaving dirty lines; median of `reps`.  A
+    plain device copy of n int32 in the same regime beside it."""
+    flush = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+    acc = torch.empty((), dtype=torch.int64, device="cuda")
+
+    def med(fn):
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(reps)]
+        torch.cuda.synchronize()
+        for a, b in evs:
+            torch.sum(flush, dim=0, dtype=torch.int64, out=acc)
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return sorted(a.elapsed_time(b) for a, b in evs)[reps // 2]
+
+    useful = 8.0 * n
+    res = {}
+    for k in CLASSES:
+        ms = med(lambda: s.run(xs[k], outs[k], stream=stream))
+        res[k] = round(useful / (ms * 1e-3) / 1e9, 1)
+    src, dst = xs["random"], torch.empty(n, dtype=torch.int32, device="cuda")
+    res["copy"] = round(useful / (med(lambda: dst.copy_(src)) * 1e-3) / 1e9, 1)
+    s.check()
+    del flush, dst
+    res["frac"] = {k: round(res[k] / PEAK_HBM_GBS, 3) for k in CLASSES}
+    res["regime"] = ("256 MB last-level cache evicted before each solve by reading a 1 GiB "
+                     "buffer (no dirty lines left); median of %d" % reps)
+    return res
 
 
 def _kernel_us_from(times):
