@@ -182,9 +182,10 @@ __device__ __forceinline__ void lb_publish_group(const LbView& V, int lvl, int g
     if (lane == 0) st_agent(const_cast<u64*>(lb_level_ptr(V, lvl)) + g, tagged(V.tag, m));
 }
 
-// Tile role: solve tile t (its open elements' -1 outputs written
-// device-coherent: an answer role may overwrite them), then publish its
-// chunk mins and, once every summary of the tile has been written, its min.
+// Tile role: solve tile t (its outputs written device-coherent: an answer
+// workgroup may overwrite the open elements' -1), then publish its chunk
+// mins and, once every output and summary of the tile has been written, its
+// min.
 template <bool INCL>
 __device__ __forceinline__ void lb_solve(const int* __restrict__ in, int* __restrict__ out,
                                          int n, int t, TileSmem& sm, const TileOut& to,
@@ -196,7 +197,7 @@ __device__ __forceinline__ void lb_solve(const int* __restrict__ in, int* __rest
     solve_tile<INCL, false, false, false, true>(out, n, t, v, pl, sm, to, L);  // ends with a barrier
     if (tid < CPT)
         st_agent(&V.CM[(size_t)t * CPT + tid], tagged(V.tag, to.cmin[(size_t)t * CPT + tid]));
-    wait_own_memory_ops();  // open list and count, open outputs, chunk mins: complete
+    wait_own_memory_ops();  // outputs, open list, spine, counts, chunk mins: complete
     __syncthreads();
     if (tid >= WAVE) return;
     int tmin = sm.wmin[0];
