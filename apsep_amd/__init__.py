@@ -41,7 +41,8 @@ EXPORTS = [
     "apsep_batch_plan_use_graph", "apsep_batch_plan_tune",
     "apsep_batch_plan_set_tile_lanes", "apsep_batch_plan_tile_lanes",
     "apsep_batch_plan_tile_elems", "apsep_batch_plan_job_tiles", "apsep_run_jobs",
-    "apsep_batch_plan_tuned",
+    "apsep_batch_plan_tuned", "apsep_batch_plan_set_persistent", "apsep_batch_plan_check",
+    "apsep_batch_plan_tuned_persistent",
     "apsep_server_create", "apsep_server_destroy", "apsep_server_submit",
     "apsep_server_poll", "apsep_server_job_info", "apsep_server_release",
     "apsep_server_drain", "apsep_server_submit_many", "apsep_server_done_upto",
@@ -123,6 +124,9 @@ def lib() -> ctypes.CDLL:
         L.apsep_run_jobs.argtypes = [vp, vp, i32, i64, vp]
         L.apsep_batch_plan_tuned.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                              ctypes.POINTER(ctypes.c_float)]
+        L.apsep_batch_plan_set_persistent.argtypes = [vp, ci]
+        L.apsep_batch_plan_check.argtypes = [vp, ctypes.POINTER(i32)]
+        L.apsep_batch_plan_tuned_persistent.argtypes = [vp, i32, ctypes.POINTER(i32)]
         L.apsep_batch_plan_tune.argtypes = [vp, vp, vp, vp, i32, i64, vp, i32,
                                             ctypes.POINTER(ctypes.c_float),
                                             ctypes.POINTER(ctypes.c_float)]
@@ -328,6 +332,28 @@ class BatchPlan:
         _check(lib().apsep_batch_plan_use_graph(self._h, int(enable)),
                "apsep_batch_plan_use_graph")
 
+    def set_persistent(self, enable: bool = True):
+        """Solve each batch in one persistent launch (True) or by the kernel
+        sequence (False, the default)."""
+        _check(lib().apsep_batch_plan_set_persistent(self._h, int(enable)),
+               "apsep_batch_plan_set_persistent")
+
+    def check(self):
+        """Blocking check of the persistent launches' device-side error word
+        (raises on error)."""
+        e = ctypes.c_int32(0)
+        _check(lib().apsep_batch_plan_check(self._h, ctypes.byref(e)), "apsep_batch_plan_check")
+        if e.value:
+            raise ApsepError(ERR_LAUNCH, "device-side error %d (grid barrier timeout)"
+                             % e.value)
+
+    def tuned_persistent(self, size_class: int) -> bool:
+        """Whether the tuner chose one persistent launch for the class."""
+        v = ctypes.c_int32(0)
+        _check(lib().apsep_batch_plan_tuned_persistent(self._h, size_class, ctypes.byref(v)),
+               "apsep_batch_plan_tuned_persistent")
+        return bool(v.value)
+
     def tuned(self, size_class: int):
         """(tile lanes, graph, ms) the tuner recorded for batches of
         2^(c-1) + 1 .. 2^c keys; lanes = 0 when the class is untuned."""
@@ -350,9 +376,10 @@ class BatchPlan:
         return v.value
 
     def tune(self, x, offsets, out, total: int | None = None, stream=None, reps: int = 3):
-        """Time both tile shapes, each by direct launches and by graph
-        replay, on this batch and keep the fastest; returns the chosen
-        shape's (eager_ms, graph_ms)."""
+        """Time every tile shape and launch sequence (kernel sequence or one
+        persistent launch), each by direct launches and by graph replay, on
+        this batch and keep the fastest; returns the chosen shape and
+        sequence's (eager_ms, graph_ms)."""
         import torch
         if total is None:
             total = x.numel()

@@ -410,6 +410,14 @@ apsep_status apsep_batch_plan_tuned(const apsep_batch_plan* p, int32_t size_clas
     return APSEP_OK;
 }
 
+apsep_status apsep_batch_plan_tuned_persistent(const apsep_batch_plan* p,
+                                               int32_t size_class_log2, int32_t* persistent) {
+    if (!p || !persistent || size_class_log2 < 0 || size_class_log2 > 31)
+        return APSEP_ERR_INVALID_ARG;
+    *persistent = p->cls_persist[size_class_log2];
+    return APSEP_OK;
+}
+
 apsep_status apsep_batch_plan_tile_elems(const apsep_batch_plan* p, int32_t* elems) {
     if (!p || !elems) return APSEP_ERR_INVALID_ARG;
     *elems = plan_tile_elems(p);
@@ -437,6 +445,26 @@ apsep_status apsep_batch_plan_use_graph(apsep_batch_plan* p, int enable) {
     return APSEP_OK;
 }
 
+static void set_persistent(apsep_batch_plan* p, int enable) {
+    if (enable != p->persistent) drop_graph(p);  // a recorded graph holds the other sequence
+    p->persistent = enable;
+}
+
+apsep_status apsep_batch_plan_set_persistent(apsep_batch_plan* p, int enable) {
+    if (!p || (enable != 0 && enable != 1)) return APSEP_ERR_INVALID_ARG;
+    set_persistent(p, enable);
+    p->tuned_auto = 0;  // an explicit mode overrides the tuned table
+    return APSEP_OK;
+}
+
+apsep_status apsep_batch_plan_check(apsep_batch_plan* p, int32_t* device_error) {
+    if (!p || !device_error) return APSEP_ERR_INVALID_ARG;
+    int code = 0;
+    const apsep_status st = apsep::single_error(p->s, &code);
+    *device_error = code;
+    return st;
+}
+
 apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
                                  const int64_t* d_offsets, int32_t* d_out, int32_t nseg,
                                  int64_t total, void* stream) {
@@ -458,6 +486,7 @@ apsep_status apsep_run_segmented(apsep_batch_plan* p, const int32_t* d_in,
     if (p->tuned_auto && p->cls_lanes[c]) {
         // the tuned shape and launch mode of this batch's size class
         set_lanes(p, p->cls_lanes[c]);
+        set_persistent(p, p->cls_persist[c]);
         if (!p->cls_graph[c]) drop_graph(p);
         p->use_graph = p->cls_graph[c];
     } else if (!p->lanes_explicit) {
@@ -503,25 +532,30 @@ apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
         return APSEP_ERR_DEVICE;
     }
     const int explicit_before = p->lanes_explicit;
-    // every tile shape x launch mode, timed on this batch
+    // every tile shape x launch sequence (kernel sequence or one persistent
+    // launch) x launch mode (direct or graph replay), timed on this batch
     constexpr int NSHAPES = 3;
     const int32_t shapes[NSHAPES] = {128, 256, 512};
-    float t[NSHAPES][2] = {};
+    float t[NSHAPES][2][2] = {};
     apsep_status st = APSEP_OK;
     for (int k = 0; k < NSHAPES && st == APSEP_OK; k++) {
         st = apsep_batch_plan_set_tile_lanes(p, shapes[k]);
-        for (int g = 0; g < 2 && st == APSEP_OK; g++) {
-            st = apsep_batch_plan_use_graph(p, g);
-            // one untimed call (records the graph), then the timed ones
-            if (st == APSEP_OK)
-                st = apsep_run_segmented(p, d_in, d_offsets, d_out, nseg, total, stream);
-            if (st == APSEP_OK && hipEventRecord(e0, hs) != hipSuccess) st = APSEP_ERR_LAUNCH;
-            for (int r = 0; r < reps && st == APSEP_OK; r++)
-                st = apsep_run_segmented(p, d_in, d_offsets, d_out, nseg, total, stream);
-            if (st == APSEP_OK &&
-                (hipEventRecord(e1, hs) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                 hipEventElapsedTime(&t[k][g], e0, e1) != hipSuccess))
-                st = APSEP_ERR_LAUNCH;
+        for (int q = 0; q < 2 && st == APSEP_OK; q++) {
+            st = apsep_batch_plan_set_persistent(p, q);
+            for (int g = 0; g < 2 && st == APSEP_OK; g++) {
+                st = apsep_batch_plan_use_graph(p, g);
+                // one untimed call (records the graph), then the timed ones
+                if (st == APSEP_OK)
+                    st = apsep_run_segmented(p, d_in, d_offsets, d_out, nseg, total, stream);
+                if (st == APSEP_OK && hipEventRecord(e0, hs) != hipSuccess) st = APSEP_ERR_LAUNCH;
+                for (int r = 0; r < reps && st == APSEP_OK; r++)
+                    st = apsep_run_segmented(p, d_in, d_offsets, d_out, nseg, total, stream);
+                if (st == APSEP_OK &&
+                    (hipEventRecord(e1, hs) != hipSuccess ||
+                     hipEventSynchronize(e1) != hipSuccess ||
+                     hipEventElapsedTime(&t[k][q][g], e0, e1) != hipSuccess))
+                    st = APSEP_ERR_LAUNCH;
+            }
         }
     }
     (void)hipEventDestroy(e0);
@@ -530,23 +564,25 @@ apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
         APSEP_CLEAR_HIP_ERROR();
         return st;
     }
-    int bk = 0, bg = 0;
+    int bk = 0, bq = 0, bg = 0;
     for (int k = 0; k < NSHAPES; k++)
-        for (int g = 0; g < 2; g++)
-            if (t[k][g] < t[bk][bg]) bk = k, bg = g;
-    if (eager_ms) *eager_ms = t[bk][0] / reps;
-    if (graph_ms) *graph_ms = t[bk][1] / reps;
+        for (int q = 0; q < 2; q++)
+            for (int g = 0; g < 2; g++)
+                if (t[k][q][g] < t[bk][bq][bg]) bk = k, bq = q, bg = g;
+    if (eager_ms) *eager_ms = t[bk][bq][0] / reps;
+    if (graph_ms) *graph_ms = t[bk][bq][1] / reps;
     st = apsep_batch_plan_set_tile_lanes(p, shapes[bk]);
-    if (st != APSEP_OK) return st;
-    // d_out holds the last shape's answer: bit-identical across shapes
-    st = apsep_batch_plan_use_graph(p, bg);
+    if (st == APSEP_OK) st = apsep_batch_plan_set_persistent(p, bq);
+    // d_out holds the last setting's answer: bit-identical across settings
+    if (st == APSEP_OK) st = apsep_batch_plan_use_graph(p, bg);
     if (st != APSEP_OK) return st;
     // remembered for the batch's size class; later batches of a tuned class
     // run with its settings
     const int c = size_class(total);
     p->cls_lanes[c] = shapes[bk];
+    p->cls_persist[c] = bq;
     p->cls_graph[c] = bg;
-    p->cls_ms[c] = t[bk][bg] / reps;
+    p->cls_ms[c] = t[bk][bq][bg] / reps;
     p->tuned_auto = 1;
     p->lanes_explicit = explicit_before;
     return APSEP_OK;

@@ -1092,7 +1092,7 @@ __device__ __forceinline__ void tile_step(const int* __restrict__ in, int* __res
 // Segmented tile t: mark the segment starts inside the tile in LDS, give
 // every lane the start of the segment holding its first element, then solve
 // with answers rebased to their segments.
-template <bool INCL>
+template <bool INCL, bool CO = false>
 __device__ __forceinline__ void tile_step_seg(const int* __restrict__ in, int* __restrict__ out,
                                               int n, int t, TileSmem& sm, const TileOut& to,
                                               const Levels& L, const SegIn& sg) {
@@ -1106,9 +1106,9 @@ __device__ __forceinline__ void tile_step_seg(const int* __restrict__ in, int* _
     const int pl = (threadIdx.x == 0 && t > 0) ? in[t * TILE - 1] : 0;
     // offsets are clamped so that malformed ones cannot send a read astray
     const int last = max(sg.nseg - 1, 0);
-    const int k0 = min(max(sg.tmap[t], 0), last);
-    const int k1 = (t + 1 < L.nt) ? min(max(sg.tmap[t + 1], k0), last) : last;
-    const int s0 = min(max(sg.tbound[t].x, 0), tile_start);
+    const int k0 = min(max(ld_i<CO>(&sg.tmap[t]), 0), last);
+    const int k1 = (t + 1 < L.nt) ? min(max(ld_i<CO>(&sg.tmap[t + 1]), k0), last) : last;
+    const int s0 = min(max(ld_i2<CO>(&sg.tbound[t]).x, 0), tile_start);
     if (tid < TILE / 32) sm.walls[tid] = 0u;
     __syncthreads();
     for (int k = k0 + 1 + tid; k <= k1; k += NT) {
@@ -1137,7 +1137,7 @@ __device__ __forceinline__ void tile_step_seg(const int* __restrict__ in, int* _
 #pragma unroll
     for (int w = 0; w < NWAVE - 1; w++)
         if (w < wave) incoming = max(incoming, sm.wlast[w]);
-    solve_tile<INCL, true, false>(out, n, t, v, pl, sm, to, L, SegLane{incoming, wm});
+    solve_tile<INCL, true, false, CO>(out, n, t, v, pl, sm, to, L, SegLane{incoming, wm});
 }
 
 // ---------------------------------------------------------------------------
@@ -1335,6 +1335,7 @@ __device__ __forceinline__ int answer_in_tile(const int* __restrict__ in, const 
 // both ends on each level, full groups one level up.  The ranges of all
 // levels follow from a and b alone, so every level's loads are issued
 // before any is used (one round, not one per level).
+template <bool CO = false>
 __device__ __forceinline__ int range_min_lane(const Levels& L, int a, int b) {
     const int lane = lane_id();
     // raw entries: level 0 as loaded, keys (levels >= 1) converted after
@@ -1353,18 +1354,18 @@ __device__ __forceinline__ int range_min_lane(const Levels& L, int a, int b) {
         const bool up_stored = lvl + 1 < L.nlev && (lvl + 1 < 2 || L.upper);
         if (!up_stored) {
             // the top stored level: the whole rest of the range
-            for (int i = a + lane; i < b; i += WAVE) mtop = min(mtop, level_entry(L, lvl, i));
+            for (int i = a + lane; i < b; i += WAVE) mtop = min(mtop, level_entry<CO>(L, lvl, i));
             done = true;
             continue;
         }
         const int ga = min(((a + FAN - 1) >> LOG_FAN) << LOG_FAN, b);
         const int gb = max((b >> LOG_FAN) << LOG_FAN, ga);
         if (lvl == 0) {
-            if (a + lane < ga) e0a = L.mn0[a + lane];
-            if (gb + lane < b) e0b = L.mn0[gb + lane];
+            if (a + lane < ga) e0a = ld_i<CO>(&L.mn0[a + lane]);
+            if (gb + lane < b) e0b = ld_i<CO>(&L.mn0[gb + lane]);
         } else {
-            if (a + lane < ga) ek[lvl][0] = key_at(L, lvl, a + lane);
-            if (gb + lane < b) ek[lvl][1] = key_at(L, lvl, gb + lane);
+            if (a + lane < ga) ek[lvl][0] = ld_u64<CO>(&key_at(L, lvl, a + lane));
+            if (gb + lane < b) ek[lvl][1] = ld_u64<CO>(&key_at(L, lvl, gb + lane));
         }
         a = ga >> LOG_FAN;
         b = gb >> LOG_FAN;
@@ -1633,20 +1634,21 @@ __device__ __forceinline__ void fix_rescan(const FixCtx& c, const Map& io, int n
 // the rest of s0's chunk, the rest of its tile's chunk mins, then whole
 // tiles through the hierarchy.  Tile s0 / TILE precedes t, so it is complete.
 static_assert(CPT <= 2 * WAVE, "the rest of a tile's chunk mins takes two loads per lane");
+template <bool CO = false>
 __device__ __forceinline__ int seg_prefix_min(const int* __restrict__ in, const TileOut& to,
                                               const Levels& L, int s0, int t) {
     const int lane = lane_id();
     const int u = s0 / TILE;
     const bool part = s0 % TILE != 0;
     // all loads first, one wave min at the end
-    const int m3 = range_min_lane(L, part ? u + 1 : u, t);
+    const int m3 = range_min_lane<CO>(L, part ? u + 1 : u, t);
     int m0 = INF, m1 = INF, m2 = INF;
     if (part) {
         const int c0 = s0 / CHUNK;
         const int cend = (u + 1) * CPT;
         if (s0 + lane < (c0 + 1) * CHUNK) m0 = in[s0 + lane];
-        if (c0 + 1 + lane < cend) m1 = to.cmin[c0 + 1 + lane];
-        if (c0 + 1 + WAVE + lane < cend) m2 = to.cmin[c0 + 1 + WAVE + lane];
+        if (c0 + 1 + lane < cend) m1 = ld_i<CO>(&to.cmin[c0 + 1 + lane]);
+        if (c0 + 1 + WAVE + lane < cend) m2 = ld_i<CO>(&to.cmin[c0 + 1 + WAVE + lane]);
     }
     return wave_min(min(min(m0, m1), min(m2, m3)));
 }
@@ -1659,11 +1661,10 @@ template <bool INCL, bool SEG = false, bool CO = false, class Map = FlatMap>
 __device__ __forceinline__ void fix_tile(const Map& io, int n, int t, const Levels& L,
                                          const TileOut& to, int* lds,
                                          const SegIn sg = SegIn{nullptr, nullptr, 0, nullptr}) {
-    static_assert(!(SEG && CO), "segmented solves run as separate launches");
     if (t == 0) return;
     const int lane = lane_id();
     const int tile_start = t * TILE;
-    const int2 tb = SEG ? sg.tbound[t] : make_int2(0, 0);
+    const int2 tb = SEG ? ld_i2<CO>(&sg.tbound[t]) : make_int2(0, 0);
     const int tw = t - WAVE + lane;  // tile whose min this lane holds
     const int mw = tw >= 0 ? ld_i<CO>(&L.mn0[tw]) : INF;
     // spines of t-1 (lanes 0..RS-1) and t-2 (lanes RS..2RS-1)
@@ -1727,7 +1728,7 @@ __device__ __forceinline__ void fix_tile(const Map& io, int n, int t, const Leve
             for (int i = lane; i < b1; i += WAVE) p = min(p, key_value(ld_u64<CO>(&key_at(L, 1, i)), L.epoch));
         }
     }
-    const int P = SEG ? seg_prefix_min(io.src(s0 / TILE), to, L, s0, t) : wave_min(p);
+    const int P = SEG ? seg_prefix_min<CO>(io.src(s0 / TILE), to, L, s0, t) : wave_min(p);
     if (!qual<INCL>(P, v1)) return;  // every open element answers -1
     int* win = lds;
     int* gm = lds + WAVE;  // min of window tiles 4g .. 4g+3

@@ -49,12 +49,14 @@ struct apsep_batch_plan {
     apsep::Page* pages;    // [tiles at capacity]: in-place job batches
     int tile_lanes;        // tile shape the batches run with: 128, 256 or 512 lanes
     int use_graph;
+    int persistent;        // segmented batches as one persistent launch
     // autotuned settings per size class (class c: batches of 2^(c-1) + 1 ..
     // 2^c keys), applied to apsep_run_segmented while `tuned_auto` is set
     int tuned_auto;
     int lanes_explicit;    // the caller chose the tile shape (no size rule)
     int cls_lanes[32];     // 0: class not tuned
     int cls_graph[32];
+    int cls_persist[32];
     float cls_ms[32];
     hipGraph_t graph;
     hipGraphExec_t exec;

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
+#include <atomic>
 #include <new>
 
 #include "apsep_device.hpp"
@@ -44,24 +45,23 @@ constexpr unsigned SEG_EPOCH = 1u;
 constexpr int SEG_FIX_WG = 256;  // four tiles per group, up to 8 waves per SIMD (as k_fix_pass)
 
 // tile -> segment holding the tile's first element, and that segment's
-// start and end; level-1 keys emptied.  A lane per segment: a wave reads 64
-// segments' offsets in one coalesced load, each lane writes the map entries
-// of its segment when it holds at most two tile starts (every short
-// segment), and the wave writes those of the longer ones together, one
-// segment at a time, its lanes over the segment's tiles (a segment of 2^20
-// keys covers 256 tiles).  Every tile start lies in exactly one segment, so
-// every map entry is written once.
-__global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__ offs,
-                                                   int nseg, int* __restrict__ tmap,
-                                                   int2* __restrict__ tbound, int nt,
-                                                   u64* __restrict__ key1, int nkey1) {
-    const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int stride = (int)(gridDim.x * blockDim.x);
-    for (int i = gid; i < nkey1; i += stride) key1[key_slot(1, i)] = ~0ull;
+// start and end.  A lane per segment: a wave reads 64 segments' offsets in
+// one coalesced load, each lane writes the map entries of its segment when
+// it holds at most two tile starts (every short segment), and the wave
+// writes those of the longer ones together, one segment at a time, its
+// lanes over the segment's tiles (a segment of 2^20 keys covers 256 tiles).
+// Every tile start lies in exactly one segment, so every map entry is
+// written once.  gw / nw: the calling wave's index among the nw waves that
+// share the work (uniform per wave).  CO: the entries are read by other
+// workgroups of the same launch (device-coherent stores).
+template <bool CO>
+__device__ __forceinline__ void seg_map_tiles(const long long* __restrict__ offs, int nseg,
+                                              int* __restrict__ tmap, int2* __restrict__ tbound,
+                                              int nt, int gw, int nw) {
     const int lane = (int)threadIdx.x & (WAVE - 1);
     const long long cap = (long long)nt * TILE;
-    for (int k0 = (gid >> 6) * WAVE; k0 < nseg; k0 += (stride >> 6) * WAVE) {  // uniform per wave
-        const int k = k0 + lane;
+    for (long long k0 = (long long)gw * WAVE; k0 < nseg; k0 += (long long)nw * WAVE) {
+        const int k = (int)k0 + lane;
         long long s = 0, e = 0;
         if (k < nseg) {
             s = max(offs[k], 0ll);
@@ -76,8 +76,8 @@ __global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__
         const bool few = t1 - t0 < 2;
         if (few) {
             for (int t = t0; t <= t1; t++) {
-                tmap[t] = k;
-                tbound[t] = b;
+                st_i<CO>(&tmap[t], k);
+                st_i2<CO>(&tbound[t], b);
             }
         }
         u64 many = __ballot(!few);
@@ -87,11 +87,22 @@ __global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__
             const int u0 = __shfl(t0, l, WAVE), u1 = __shfl(t1, l, WAVE);
             const int2 ub = make_int2(__shfl(b.x, l, WAVE), __shfl(b.y, l, WAVE));
             for (int t = u0 + lane; t <= u1; t += WAVE) {
-                tmap[t] = k0 + l;
-                tbound[t] = ub;
+                st_i<CO>(&tmap[t], (int)k0 + l);
+                st_i2<CO>(&tbound[t], ub);
             }
         }
     }
+}
+
+// The tile map and bounds (above), and level-1 keys emptied.
+__global__ __launch_bounds__(256) void k_seg_setup(const long long* __restrict__ offs,
+                                                   int nseg, int* __restrict__ tmap,
+                                                   int2* __restrict__ tbound, int nt,
+                                                   u64* __restrict__ key1, int nkey1) {
+    const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int stride = (int)(gridDim.x * blockDim.x);
+    for (int i = gid; i < nkey1; i += stride) key1[key_slot(1, i)] = ~0ull;
+    seg_map_tiles<false>(offs, nseg, tmap, tbound, nt, gid >> 6, stride >> 6);
 }
 
 // 256-lane tiles: 5 waves per SIMD (88 VGPRs, no spill; 5 tiles per CU)
@@ -127,6 +138,138 @@ __global__ __launch_bounds__(SEG_FIX_WG) __attribute__((amdgpu_waves_per_eu(8)))
     const int t = (int)(blockIdx.x * (SEG_FIX_WG / WAVE) + (threadIdx.x >> 6));
     if (t >= L.nt) return;
     fix_tile<INCL, true>(FlatMap{in, out}, n, t, L, to, lds[threadIdx.x >> 6], sg);
+}
+
+// ---- The whole batch in one persistent launch ----------------------------
+// The four steps above in one launch of co-resident workgroups, separated
+// by grid barriers (k_seg_persist_begin clears the barrier words on the
+// stream first, so the pair can be captured into a graph).  Which
+// workgroups run together is found at run time (grid_join / grid_count):
+// participant `me` of `np` maps segments me*NWAVE.. (a lane each), then
+// solves tiles me, me + np, ..., builds level-1 keys with its waves, and
+// fixes the tiles it solved, one wave per tile, so each tile's outputs and
+// open list stay in the XCD (L2) that wrote them.  Everything another
+// workgroup reads after a barrier (map and bounds, tile and chunk mins,
+// spines, counts, keys) is written and read device-coherent, so the
+// barriers need no cache write-back.  Level 1 is built whole (no reset);
+// levels >= 2 one per barrier.
+constexpr int PERSIST_WAVES_PER_EU = 4;
+static_assert(NWAVE * FIX_LDS_INTS * sizeof(int) <= sizeof(TileSmem), "fix-up LDS");
+
+__global__ __launch_bounds__(BAR_SUB) void k_seg_persist_begin(GridBarrier* bar) {
+    grid_barrier_reset(bar);
+}
+
+template <bool INCL>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PERSIST_WAVES_PER_EU)))
+void k_seg_persist(const int* __restrict__ in, int* __restrict__ out, int n, TileOut to,
+                   Levels L, SegIn sg, int* __restrict__ tmap, int2* __restrict__ tbound,
+                   GridBarrier* bar) {
+    __shared__ TileSmem sm;
+    const int j = grid_join(bar);
+    if (j < 0) return;  // uniform per workgroup: not a participant
+    const int c = (int)(blockIdx.x % SPT_SLOTS);
+    const u64 pub = grid_count(bar, c == 0 && j == 0);
+    if (pub == 0ull) return;  // timed out (error word raised)
+    int me = j, np = 0;
+#pragma unroll
+    for (int k = 0; k < SPT_SLOTS; k++) {
+        const int nk = (int)((pub >> (8 * k)) & 0xFF);
+        if (k < c) me += nk;
+        np += nk;
+    }
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = lane_id();
+    const bool leader = threadIdx.x == 0;
+    const int nt = L.nt;
+    unsigned gen = grid_barrier_start(bar), jb = 0;  // before this workgroup arrives
+    seg_map_tiles<true>(sg.offs, sg.nseg, tmap, tbound, nt, me * NWAVE + wave, np * NWAVE);
+    if (!grid_barrier_co(bar, (unsigned)np, (unsigned)me, gen, jb, leader)) return;
+    for (int t = me; t < nt; t += np) {
+        tile_step_seg<INCL, true>(in, out, n, t, sm, to, L, sg);
+        __syncthreads();  // the next tile reuses the LDS
+    }
+    if (nt <= 1) return;  // uniform across the grid
+    if (!grid_barrier_co(bar, (unsigned)np, (unsigned)me, gen, jb, leader)) return;
+    if (L.nlev > 1) {
+        for (int g = me * NWAVE + wave; g < L.n[1]; g += np * NWAVE) {  // uniform per wave
+            const int idx = g * FAN + lane;
+            const int m = wave_min(idx < nt ? ld_i<true>(&L.mn0[idx]) : INF);
+            if (lane == 0) st_u64<true>(&key_at(L, 1, g), make_key(L.epoch, m));
+        }
+        if (L.upper) {
+            for (int l = 2; l < L.nlev; l++) {
+                if (!grid_barrier_co(bar, (unsigned)np, (unsigned)me, gen, jb, leader)) return;
+                for (int g = me * NWAVE + wave; g < L.n[l]; g += np * NWAVE) {
+                    const int idx = g * FAN + lane;
+                    const int x = idx < L.n[l - 1]
+                                      ? key_value(ld_u64<true>(&key_at(L, l - 1, idx)), L.epoch)
+                                      : INF;
+                    const int m = wave_min(x);
+                    if (lane == 0) st_u64<true>(&key_at(L, l, g), make_key(L.epoch, m));
+                }
+            }
+        }
+        if (!grid_barrier_co(bar, (unsigned)np, (unsigned)me, gen, jb, leader)) return;
+    }
+    int* lds = reinterpret_cast<int*>(&sm) + wave * FIX_LDS_INTS;
+    for (long long t = me + (long long)wave * np; t < nt; t += (long long)NWAVE * np)
+        fix_tile<INCL, true, true>(FlatMap{in, out}, n, (int)t, L, to, lds, sg);
+}
+
+// Workgroups of k_seg_persist that fit on one CU at once (the smaller of
+// the two predicates' occupancy, never above what the waves-per-EU
+// attribute guarantees); 0 if the runtime cannot tell.
+static int persist_per_cu() {
+    static std::atomic<int> cached{-1};
+    int v = cached.load(std::memory_order_relaxed);
+    if (v >= 0) return v;
+    int a = 0, b = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_seg_persist<false>, NT, 0);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_seg_persist<true>, NT, 0);
+    if (e != hipSuccess) {
+        APSEP_CLEAR_HIP_ERROR();
+        return 0;
+    }
+    v = min(min(a, b), PERSIST_WAVES_PER_EU * 4 / NWAVE);
+    cached.store(v < 0 ? 0 : v, std::memory_order_relaxed);
+    return v < 0 ? 0 : v;
+}
+
+template <bool INCL>
+static apsep_status seg_launch_persist(apsep_batch_plan* p, const int* in,
+                                       const long long* offs, int* out, int nseg, int n,
+                                       hipStream_t st) {
+    apsep_scratch* s = p->s;
+    const int nt = (int)(((long long)n + TILE - 1) / TILE);
+    const int per_cu = persist_per_cu();
+    if (per_cu <= 0) return APSEP_ERR_UNSUPPORTED;
+    Levels L;
+    int sz[MAX_LEVELS];
+    L.nt = nt;
+    L.nlev = level_sizes(nt, sz);
+    L.mn0 = s->mn0;
+    L.upper = (L.nlev > 2 && sz[1] > UPPER_MIN) ? 1 : 0;
+    for (int l = 0; l < MAX_LEVELS; l++) {
+        L.n[l] = l < L.nlev ? sz[l] : 0;
+        L.key[l] = (l >= 1 && l < L.nlev) ? s->key[l] : nullptr;
+    }
+    L.epoch = SEG_EPOCH;
+    TileOut to{s->cmin, s->qcount, s->qlist, s->scount, s->spine};
+    SegIn sg{offs, p->tmap, nseg, p->tbound};
+    // a workgroup per tile, or per NWAVE * 64 segments, up to what runs at
+    // once (and what the join's per-slot counts hold)
+    long long want = (nseg + (long long)NWAVE * WAVE - 1) / ((long long)NWAVE * WAVE);
+    if (want < nt) want = nt;
+    long long grid = (long long)per_cu * s->num_cus;
+    if (grid > (long long)SPT_SLOTS * SPT_SLOT_MAX) grid = (long long)SPT_SLOTS * SPT_SLOT_MAX;
+    if (grid > want) grid = want;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_seg_persist_begin, dim3(1), dim3(BAR_SUB), 0, st, s->bar);
+    hipLaunchKernelGGL(k_seg_persist<INCL>, dim3((unsigned)grid), dim3(NT), 0, st, in, out, n,
+                       to, L, sg, p->tmap, p->tbound, s->bar);
+    return APSEP_LAUNCH_FAILED() ? APSEP_ERR_LAUNCH : APSEP_OK;
 }
 
 template <bool INCL>
@@ -295,6 +438,9 @@ apsep_status jobs_run(apsep_batch_plan* p, const void* jobs, int njobs, int tile
 apsep_status seg_run(apsep_batch_plan* p, const int32_t* in, const int64_t* offs,
                      int32_t* out, int nseg, int n, hipStream_t st) {
     const long long* o = reinterpret_cast<const long long*>(offs);
+    if (p->persistent)
+        return p->s->incl ? seg_launch_persist<true>(p, in, o, out, nseg, n, st)
+                          : seg_launch_persist<false>(p, in, o, out, nseg, n, st);
     return p->s->incl ? seg_launch<true>(p, in, o, out, nseg, n, st)
                       : seg_launch<false>(p, in, o, out, nseg, n, st);
 }

@@ -554,8 +554,9 @@ def bench_tune(args, torch, apsep_amd):
     """The autotuner's choices per size class: for batches of about 2^c keys
     (c = 14 .. 28; log-uniform segment lengths 64 .. min(2^20, 2^c / 4),
     class per segment uniform over random / descending / ascending), every
-    tile shape x launch mode timed (median of `steps` runs), and what
-    apsep_batch_plan_tune keeps for the class."""
+    tile shape x launch sequence (kernel sequence, or one persistent launch:
+    "p") x launch mode (e = direct, g = graph replay) timed (median of
+    `steps` runs), and what apsep_batch_plan_tune keeps for the class."""
     import numpy as np
     rows = []
     stream = torch.cuda.Stream()
@@ -572,21 +573,26 @@ def bench_tune(args, torch, apsep_amd):
                "segments": nseg, "times_ms": {}}
         with apsep_amd.BatchPlan(total, nseg) as p, torch.cuda.stream(stream):
             for lanes in (128, 256, 512):
-                for graph in (0, 1):
-                    p.set_tile_lanes(lanes)
-                    p.use_graph(bool(graph))
-                    run = lambda: p.run(x, offs, out, total=total, stream=stream)
-                    run()
-                    run()
-                    ts = []
-                    for _ in range(args.steps):
-                        ts.append(_timed(torch, stream, run, 1))
-                    row["times_ms"]["%d%s" % (lanes, "g" if graph else "e")] = \
-                        round(sorted(ts)[len(ts) // 2], 4)
+                for persist in (0, 1):
+                    for graph in (0, 1):
+                        p.set_tile_lanes(lanes)
+                        p.set_persistent(bool(persist))
+                        p.use_graph(bool(graph))
+                        run = lambda: p.run(x, offs, out, total=total, stream=stream)
+                        run()
+                        run()
+                        ts = []
+                        for _ in range(args.steps):
+                            ts.append(_timed(torch, stream, run, 1))
+                        row["times_ms"]["%d%s%s" % (lanes, "p" if persist else "",
+                                                    "g" if graph else "e")] = \
+                            round(sorted(ts)[len(ts) // 2], 4)
+            p.check()
             p.tune(x, offs, out, total=total, stream=stream, reps=max(3, args.steps // 2))
             cls = int(total - 1).bit_length()  # the library's size class of `total`
             lanes, graph, ms = p.tuned(cls)
-            row["tuned"] = {"lanes": lanes, "graph": bool(graph), "ms": round(ms, 4)}
+            row["tuned"] = {"lanes": lanes, "persistent": p.tuned_persistent(cls),
+                            "graph": bool(graph), "ms": round(ms, 4)}
         best = min(row["times_ms"], key=row["times_ms"].get)
         row["best"] = best
         row["best_GBps"] = round(8.0 * total / (row["times_ms"][best] * 1e-3) / 1e9, 1)
@@ -595,7 +601,7 @@ def bench_tune(args, torch, apsep_amd):
         del x, out, offs
         torch.cuda.empty_cache()
     print(json.dumps({
-        "metric": "segmented batch time per size class, every tile shape x launch mode",
+        "metric": "segmented batch time per size class, every tile shape x launch sequence x mode",
         "value": None, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 2,
         "ms_per_step": None, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "int32",

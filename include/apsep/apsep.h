@@ -226,13 +226,34 @@ apsep_status apsep_batch_plan_use_graph(apsep_batch_plan* p, int enable);
 apsep_status apsep_batch_plan_set_tile_lanes(apsep_batch_plan* p, int32_t lanes);
 apsep_status apsep_batch_plan_tile_lanes(const apsep_batch_plan* p, int32_t* lanes);
 
-/* Autotune a plan for a batch: for each tile shape, times `reps`
- * direct-launch runs and `reps` graph replays of this batch on `stream`
- * (non-null); keeps the fastest shape and mode and reports that shape's two
+/* One persistent launch per batch (enable = 1) instead of the kernel
+ * sequence (0, the default): co-resident workgroups map the segments, solve
+ * the tiles, build the hierarchy and fix up the open elements, with grid
+ * barriers between the steps, after a one-workgroup kernel that clears the
+ * barrier (two launches in all, both capturable: with graphs on, the pair is
+ * recorded and replayed).  Answers do not depend on it.  No counterpart in
+ * the reference, whose persistent kernel (runSPT, src/apsep.cuh:3052-3067)
+ * solves one array.  It drops a recorded graph and switches the tuned table
+ * off, like apsep_batch_plan_use_graph.  apsep_run_jobs ignores it (job
+ * batches always run the kernel sequence).  Workgroups that cannot start
+ * while the others run (a GPU shared with other work) leave at once and the
+ * rest share their work; a barrier that still waits ~1 s raises the device
+ * error that apsep_batch_plan_check reports, and the output is then
+ * invalid. */
+apsep_status apsep_batch_plan_set_persistent(apsep_batch_plan* p, int enable);
+
+/* Device-side error of the plan's persistent launches (as
+ * apsep_scratch_check: 0 = none; blocking; the word is cleared once read). */
+apsep_status apsep_batch_plan_check(apsep_batch_plan* p, int32_t* device_error);
+
+/* Autotune a plan for a batch: for each tile shape and each launch sequence
+ * (kernel sequence, one persistent launch), times `reps` direct-launch runs
+ * and `reps` graph replays of this batch on `stream` (non-null); keeps the
+ * fastest shape, sequence and mode and reports that shape and sequence's two
  * mean times (blocking; d_out holds the answer afterwards).  The choice is
  * also recorded for the batch's size class (c with 2^(c-1) < total <= 2^c):
  * from then on apsep_run_segmented runs every batch of a tuned class with
- * that class's shape and mode. */
+ * that class's shape, sequence and mode. */
 apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
                                    const int64_t* d_offsets, int32_t* d_out, int32_t nseg,
                                    int64_t total, void* stream, int32_t reps,
@@ -242,6 +263,9 @@ apsep_status apsep_batch_plan_tune(apsep_batch_plan* p, const int32_t* d_in,
  * when the class has not been tuned), graph mode and its mean time in ms. */
 apsep_status apsep_batch_plan_tuned(const apsep_batch_plan* p, int32_t size_class_log2,
                                     int32_t* lanes, int32_t* graph, float* ms);
+/* The tuned launch sequence of a size class: 1 = one persistent launch. */
+apsep_status apsep_batch_plan_tuned_persistent(const apsep_batch_plan* p,
+                                               int32_t size_class_log2, int32_t* persistent);
 
 /* In-place job batches: many independent arrays solved by one launch
  * sequence, each read where it lies and answered into a buffer of the

@@ -77,6 +77,10 @@ def test_run_and_launch_argument_checks():
     L = A.lib()
     assert L.apsep_run(None, 16, 32, 4, None) == A.ERR_INVALID_ARG
     assert L.apsep_scratch_set_persistent_blocks(None, 8) == A.ERR_INVALID_ARG
+    assert L.apsep_batch_plan_set_persistent(None, 1) == A.ERR_INVALID_ARG
+    code = ctypes.c_int32(0)
+    assert L.apsep_batch_plan_check(None, ctypes.byref(code)) == A.ERR_INVALID_ARG
+    assert L.apsep_batch_plan_tuned_persistent(None, 3, ctypes.byref(code)) == A.ERR_INVALID_ARG
     # n <= 0 is a no-op, as the reference's run*/launch* return early
     assert L.apsep_launch(1, None, None, 0, 0) == A.OK
     assert L.apsep_launch(1, None, None, -5, 0) == A.OK

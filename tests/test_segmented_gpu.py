@@ -320,6 +320,14 @@ def test_c4_full_batch_eager_and_graph(torch_mod):
             _log("C4 eager", lanes)
             p.run(x, o, out, stream=stream)
             check("eager lanes=%d" % lanes)
+        # one persistent launch (apsep_batch_plan_set_persistent)
+        p.set_persistent(True)
+        out.fill_(int(POISON))
+        _log("C4 persistent", p.tile_lanes)
+        p.run(x, o, out, stream=stream)
+        check("persistent lanes=%d" % p.tile_lanes)
+        p.check()
+        p.set_persistent(False)
         p.tune(x, o, out, stream=stream)
         check("autotuned")
         p.use_graph(True)
@@ -357,6 +365,7 @@ def test_tuned_size_classes(torch_mod):
         classes = [c for c in range(32) if tuned[c][0]]
         _log("tuned classes", [(c, tuned[c]) for c in classes])
         assert len(classes) == 2 and all(tuned[c][0] in (128, 256, 512) for c in classes)
+        _log("tuned sequences", [(c, p.tuned_persistent(c)) for c in classes])
         for rep in range(2):
             for i, (a, off) in enumerate(batches):  # the third class is untuned
                 _log("class run", rep, i, a.shape[0])
