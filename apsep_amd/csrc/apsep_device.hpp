@@ -1760,9 +1760,10 @@ __device__ __forceinline__ void fix_tile(const Map& io, int n, int t, const Leve
 // over 64 sub-counters (one 128-byte line each); the last arriver of a
 // sub-group arrives at the root; the last root arriver bumps the generation
 // word (its own line).  The counters only grow inside a launch (barrier j
-// of a launch is complete for a sub-group at (j + 1) x its size), and a
-// one-workgroup kernel (grid_barrier_reset) zeroes them before every launch,
-// so a launch cut short by a timeout leaves nothing behind for the next.
+// of a launch is complete for a sub-group at (j + 1) x its size), and they
+// are zeroed (grid_barrier_reset) before every launch, by the previous eager
+// launch or by a one-workgroup kernel (see BAR_SETS), so a launch cut short
+// by a timeout leaves nothing behind for the next.
 // Only the workgroups that grid_join found running take part, so every
 // participant reaches every barrier.  The spin is still bounded: any stall
 // raises `err` and lets every workgroup leave instead of hanging the
@@ -1801,19 +1802,37 @@ constexpr unsigned DISC_CLOSED = 1u << 31;
 // idle GPU)
 constexpr unsigned long long DISC_WAIT_TICKS = 5000;
 
-// Launched (one workgroup of BAR_SUB lanes) before each persistent launch.
+// Clears a barrier's counters (the first BAR_SUB threads of a workgroup;
+// device-coherent stores, like the atomics that use them).
 __device__ __forceinline__ void grid_barrier_reset(GridBarrier* b) {
     const int i = (int)threadIdx.x;
-    if (i < BAR_SUB) b->sub[i * BAR_LINE] = 0u;
+    constexpr int sc = __HIP_MEMORY_SCOPE_AGENT;
+    if (i < BAR_SUB) __hip_atomic_store(&b->sub[i * BAR_LINE], 0u, __ATOMIC_RELAXED, sc);
     if (i < SPT_SLOTS) {
-        b->disc[i * BAR_LINE] = 0u;
-        b->tiles[i * BAR_LINE] = 0u;
+        __hip_atomic_store(&b->disc[i * BAR_LINE], 0u, __ATOMIC_RELAXED, sc);
+        __hip_atomic_store(&b->tiles[i * BAR_LINE], 0u, __ATOMIC_RELAXED, sc);
     }
     if (i == 0) {
-        b->root = 0u;
-        b->pub = 0ull;
-        b->orphans = 0u;
+        __hip_atomic_store(&b->root, 0u, __ATOMIC_RELAXED, sc);
+        __hip_atomic_store(&b->pub, 0ull, __ATOMIC_RELAXED, sc);
+        __hip_atomic_store(&b->orphans, 0u, __ATOMIC_RELAXED, sc);
     }
+}
+
+// A scratch holds BAR_SETS barriers.  Eager persistent launches alternate
+// between sets 0 and 1 (the host keeps the parity): each launch's workgroup
+// 0 clears the other set for the next one (grid_barrier_clear_next), so no
+// launch of its own is needed.  That is safe because no workgroup of a
+// launch touches the other set, every workgroup of a launch runs (late ones
+// join, see the close and leave), and the launch that used the other set
+// last ended before this one started.  Captured launches use set
+// BAR_CAPTURED, cleared by a one-workgroup kernel recorded before them
+// (a replay has no host to flip a parity).
+constexpr int BAR_SETS = 3;
+constexpr int BAR_CAPTURED = 2;
+
+__device__ __forceinline__ void grid_barrier_clear_next(GridBarrier* next) {
+    if (next && blockIdx.x == 0) grid_barrier_reset(next);
 }
 
 // Which workgroups of a normally launched persistent grid run at the same

@@ -29,7 +29,8 @@ struct apsep_scratch {
     int spt_grid;        // SPT persistent grid size (occupancy x CUs)
     int spt_blocks;      // SPT grid the caller asked for (0: spt_grid)
     int* spt_link;       // SPT: per tile, the next tile of a workgroup's overflow list
-    apsep::GridBarrier* bar;
+    apsep::GridBarrier* bar;   // BAR_SETS barriers (persistent launches)
+    int bar_parity;            // the set the next eager persistent launch uses
     // look-back runs: control words and the published (tagged) words
     apsep::LbCtl* lb_ctl;
     unsigned long long* lb_M;

@@ -142,8 +142,9 @@ __global__ __launch_bounds__(SEG_FIX_WG) __attribute__((amdgpu_waves_per_eu(8)))
 
 // ---- The whole batch in one persistent launch ----------------------------
 // The four steps above in one launch of co-resident workgroups, separated
-// by grid barriers (k_seg_persist_begin clears the barrier words on the
-// stream first, so the pair can be captured into a graph).  Which
+// by grid barriers (an eager launch clears the barrier set of the next one;
+// a captured launch is preceded by k_seg_persist_begin, which clears its
+// own, so the pair can be replayed).  Which
 // workgroups run together is found at run time (grid_join / grid_count):
 // participant `me` of `np` maps segments me*NWAVE.. (a lane each), then
 // solves tiles me, me + np, ..., builds level-1 keys with its waves, and
@@ -164,8 +165,9 @@ template <bool INCL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PERSIST_WAVES_PER_EU)))
 void k_seg_persist(const int* __restrict__ in, int* __restrict__ out, int n, TileOut to,
                    Levels L, SegIn sg, int* __restrict__ tmap, int2* __restrict__ tbound,
-                   GridBarrier* bar) {
+                   GridBarrier* bar, GridBarrier* next) {
     __shared__ TileSmem sm;
+    grid_barrier_clear_next(next);
     const int j = grid_join(bar);
     if (j < 0) return;  // uniform per workgroup: not a participant
     const int c = (int)(blockIdx.x % SPT_SLOTS);
@@ -266,10 +268,30 @@ static apsep_status seg_launch_persist(apsep_batch_plan* p, const int* in,
     if (grid > (long long)SPT_SLOTS * SPT_SLOT_MAX) grid = (long long)SPT_SLOTS * SPT_SLOT_MAX;
     if (grid > want) grid = want;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(k_seg_persist_begin, dim3(1), dim3(BAR_SUB), 0, st, s->bar);
+    // barrier sets as the single-array persistent solve's: a captured run
+    // clears its own set with a kernel, an eager run the next run's set
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+        APSEP_CLEAR_HIP_ERROR();
+        return APSEP_ERR_UNSUPPORTED;
+    }
+    if (cs != hipStreamCaptureStatusNone && cs != hipStreamCaptureStatusActive)
+        return APSEP_ERR_UNSUPPORTED;
+    const bool cap = cs == hipStreamCaptureStatusActive;
+    GridBarrier* bar;
+    GridBarrier* next = nullptr;
+    if (cap) {
+        bar = s->bar + BAR_CAPTURED;
+        hipLaunchKernelGGL(k_seg_persist_begin, dim3(1), dim3(BAR_SUB), 0, st, bar);
+    } else {
+        bar = s->bar + s->bar_parity;
+        next = s->bar + (1 - s->bar_parity);
+    }
     hipLaunchKernelGGL(k_seg_persist<INCL>, dim3((unsigned)grid), dim3(NT), 0, st, in, out, n,
-                       to, L, sg, p->tmap, p->tbound, s->bar);
-    return APSEP_LAUNCH_FAILED() ? APSEP_ERR_LAUNCH : APSEP_OK;
+                       to, L, sg, p->tmap, p->tbound, bar, next);
+    if (APSEP_LAUNCH_FAILED()) return APSEP_ERR_LAUNCH;
+    if (!cap) s->bar_parity ^= 1;
+    return APSEP_OK;
 }
 
 template <bool INCL>

@@ -45,7 +45,8 @@ __global__ __launch_bounds__(256) void k_reset_level1(u64* __restrict__ key1, in
         key1[key_slot(1, i)] = ~0ull;
 }
 
-// Before every SPT launch: the grid barrier's arrival counters zeroed.
+// Before every captured SPT launch: its grid barrier's counters zeroed (an
+// eager launch finds its set cleared by the eager launch before it).
 __global__ __launch_bounds__(BAR_SUB) void k_spt_begin(GridBarrier* bar) {
     grid_barrier_reset(bar);
 }
@@ -100,13 +101,14 @@ constexpr int SPT_DYN_CAP = 32;  // tiles taken by ticket listed in LDS
 template <bool INCL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SPT_WAVES_PER_EU)))
 void k_spt(const int* __restrict__ in, int* __restrict__ out, int n,
-           TileOut to, Levels L, GridBarrier* bar, int* link) {
+           TileOut to, Levels L, GridBarrier* bar, GridBarrier* next, int* link) {
     __shared__ TileSmem sm;
     __shared__ int s_dyn[SPT_DYN_CAP];  // tiles this workgroup took by ticket
     __shared__ int s_head;              // and the last one listed through link
     __shared__ unsigned s_next;         // the next tile's ticket
     static_assert(NWAVE * FIX_LDS_INTS * sizeof(int) <= sizeof(TileSmem), "fix-up LDS");
     if (threadIdx.x == 0) s_head = -1;  // (grid_join's barrier orders it)
+    grid_barrier_clear_next(next);
     const int j = grid_join(bar);
     if (j < 0) return;  // uniform per workgroup: not a participant
     // Nothing per-lane stays live across the tile loop (it would be spilled
@@ -306,7 +308,7 @@ constexpr int64_t common_scratch_bytes(int64_t n_cap, int tile) {
     b += ((int64_t)nt * QS + 2) * 8;                  // qlist
     b += (round4(nt) + 4) * 4;                        // scount
     b += ((int64_t)nt * RS + 2) * 8;                  // spine
-    b += (int64_t)sizeof(GridBarrier);                // grid barrier
+    b += (int64_t)sizeof(GridBarrier) * BAR_SETS;     // grid barriers
     return b;
 }
 
@@ -348,7 +350,7 @@ static bool carve_arrays(apsep_scratch* s) {
     s->qlist = (int2*)carve(p, ((int64_t)nt * QS + 2) * 8);
     s->scount = (int*)carve(p, (round4(nt) + 4) * 4);
     s->spine = (int2*)carve(p, ((int64_t)nt * RS + 2) * 8);
-    s->bar = (GridBarrier*)carve(p, (int64_t)sizeof(GridBarrier));
+    s->bar = (GridBarrier*)carve(p, (int64_t)sizeof(GridBarrier) * BAR_SETS);
     if (s->algo == APSEP_ALGO_SPT) s->spt_link = (int*)carve(p, (round4(nt) + 4) * 4);
     if (s->algo == APSEP_ALGO_LOOKBACK) {
         s->lb_ctl = (LbCtl*)carve(p, (int64_t)sizeof(LbCtl));
@@ -489,11 +491,20 @@ static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
     if (grid > nt) grid = nt;
     if (grid > SPT_SLOTS * SPT_SLOT_MAX) grid = SPT_SLOTS * SPT_SLOT_MAX;
     if (grid < 1) grid = 1;
-    GridBarrier* bar = s->bar;
-    hipLaunchKernelGGL(k_spt_begin, dim3(1), dim3(BAR_SUB), 0, st, bar);
-    hipLaunchKernelGGL(k_spt<INCL>, dim3(grid), dim3(NT), 0, st, in, out, n, to, L, bar,
+    GridBarrier* bar;
+    GridBarrier* next = nullptr;
+    if (cap) {  // a replay clears its own set first
+        bar = s->bar + BAR_CAPTURED;
+        hipLaunchKernelGGL(k_spt_begin, dim3(1), dim3(BAR_SUB), 0, st, bar);
+    } else {    // this launch clears the set the next one uses
+        bar = s->bar + s->bar_parity;
+        next = s->bar + (1 - s->bar_parity);
+    }
+    hipLaunchKernelGGL(k_spt<INCL>, dim3(grid), dim3(NT), 0, st, in, out, n, to, L, bar, next,
                        s->spt_link);
-    return APSEP_LAUNCH_FAILED() ? APSEP_ERR_LAUNCH : APSEP_OK;
+    if (APSEP_LAUNCH_FAILED()) return APSEP_ERR_LAUNCH;
+    if (!cap) s->bar_parity ^= 1;
+    return APSEP_OK;
 }
 
 // Launch r of each kernel solves array r % m (its own scratch, input and
@@ -662,20 +673,25 @@ apsep_status single_error(apsep_scratch* s, int* code) {
         }
     }
     if (!s->bar) return APSEP_OK;
-    int err = 0;
-    if (hipMemcpy(&err, &s->bar->err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
-        APSEP_CLEAR_HIP_ERROR();
-        return APSEP_ERR_LAUNCH;
-    }
-    if (err) {
-        // cleared once read (the counters are zeroed before every launch)
-        const int zero = 0;
-        if (hipMemcpy(&s->bar->err, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    int any = 0;
+    for (int k = 0; k < BAR_SETS; k++) {
+        int err = 0;
+        if (hipMemcpy(&err, &s->bar[k].err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
             APSEP_CLEAR_HIP_ERROR();
             return APSEP_ERR_LAUNCH;
         }
+        if (err) {
+            // cleared once read (the counters are cleared before every launch)
+            const int zero = 0;
+            if (hipMemcpy(&s->bar[k].err, &zero, sizeof(int), hipMemcpyHostToDevice) !=
+                hipSuccess) {
+                APSEP_CLEAR_HIP_ERROR();
+                return APSEP_ERR_LAUNCH;
+            }
+            any = err;
+        }
     }
-    *code = err;
+    *code = any;
     return APSEP_OK;
 }
 

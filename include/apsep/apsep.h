@@ -229,9 +229,9 @@ apsep_status apsep_batch_plan_tile_lanes(const apsep_batch_plan* p, int32_t* lan
 /* One persistent launch per batch (enable = 1) instead of the kernel
  * sequence (0, the default): co-resident workgroups map the segments, solve
  * the tiles, build the hierarchy and fix up the open elements, with grid
- * barriers between the steps, after a one-workgroup kernel that clears the
- * barrier (two launches in all, both capturable: with graphs on, the pair is
- * recorded and replayed).  Answers do not depend on it.  No counterpart in
+ * barriers between the steps (a captured run is preceded by a
+ * one-workgroup kernel that clears its barrier, an eager run needs none;
+ * with graphs on, the pair is recorded and replayed).  Answers do not depend on it.  No counterpart in
  * the reference, whose persistent kernel (runSPT, src/apsep.cuh:3052-3067)
  * solves one array.  It drops a recorded graph and switches the tuned table
  * off, like apsep_batch_plan_use_graph.  apsep_run_jobs ignores it (job
