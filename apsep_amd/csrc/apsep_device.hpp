@@ -1232,6 +1232,14 @@ struct LbCtl {
     unsigned pad1[31];
 };
 constexpr int LB_LEVELS = MAX_LEVELS;
+// Look-back group counters: eager runs alternate between sets 0 and 1 and
+// clear the other for the next run; captured runs use set 2, cleared by the
+// kernel recorded before them.  Eager runs take run tags from the host
+// (1 .. LB_TAG_CAPTURED_MIN - 1), captured runs from a device counter
+// (LB_TAG_CAPTURED_MIN and up), so the two never share a tag.
+constexpr int LB_GC_SETS = 3;
+constexpr int LB_GC_CAPTURED = 2;
+constexpr unsigned LB_TAG_CAPTURED_MIN = 0x80000000u;
 
 // per fix-up wave: tile-min window (64), its 16 four-tile group mins, two
 // spines (2 x RS int2)

@@ -34,7 +34,11 @@ struct apsep_scratch {
     // look-back runs: control words and the published (tagged) words
     apsep::LbCtl* lb_ctl;
     unsigned long long* lb_M;
-    unsigned* lb_GC;
+    unsigned* lb_GC;           // LB_GC_SETS sets of group counters, lb_gc_stride apart
+    int64_t lb_gc_stride;
+    int64_t lb_m_bytes, lb_cm_bytes, lb_g_bytes[apsep::LB_LEVELS];  // published arrays
+    unsigned lb_tag;           // the last eager run's tag (LB_TAG_CAPTURED_MIN and up: captured runs)
+    int lb_parity;             // the counter set the next eager run uses
     unsigned long long* lb_CM;
     unsigned long long* lb_G[apsep::LB_LEVELS];
 };

@@ -26,10 +26,11 @@
 // complete value or recognisably stale; the open list and the outputs the
 // answers overwrite are written device-coherent, all before the tile's M.
 // A reader that meets a stale word polls it (bounded: past the limit the
-// run is flagged as failed and every wait ends).  The run tag lives in
-// device memory and is advanced by a one-workgroup kernel before the solve,
-// so the launch sequence holds no host-baked state and can be captured
-// into a graph.
+// run is flagged as failed and every wait ends).  An eager run takes its
+// tag from the host and clears the group counters of the next eager run
+// (two sets in turn); a captured run's tag lives in device memory and is
+// advanced, with its own counter set cleared, by a one-workgroup kernel
+// recorded before the solve, so a replay holds no host-baked state.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -58,9 +59,11 @@ struct LbView {
     u64* G[LB_LEVELS];        // G[0] unused; G[l] = min of 64^l tiles
     unsigned* GC;             // group counters: entries of group g at level l - 1
                               // published, at GC[gn[1] + .. + gn[l - 1] + g]
+    unsigned* GC_next;        // the next eager run's set, cleared by this run (or null)
+    int ngc;                  // group counters in a set
     int gn[LB_LEVELS];        // entries per level (gn[0] = nt)
     u64* CM;                  // [nt * CPT] chunk mins
-    unsigned tag;
+    unsigned tag;             // 0: read the captured run's tag from ctl
     int nt;
 };
 
@@ -68,12 +71,16 @@ struct LbView {
 __device__ __forceinline__ int lb_wait(const LbView& V, const u64* p) {
     u64 w = ld_agent(p);
     unsigned it = 0;
+    // the error word holds the tag of the run that gave up: an older run's
+    // failure does not end this run's waits
     while ((unsigned)(w >> 32) != V.tag) {
         if (++it > LB_SPIN_LIMIT) {
-            __hip_atomic_store(&V.ctl->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&V.ctl->err, (int)V.tag, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
             return INF;
         }
-        if (__hip_atomic_load(&V.ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        if (__hip_atomic_load(&V.ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+            (int)V.tag)
             return INF;
         __builtin_amdgcn_s_sleep(2);
         w = ld_agent(p);
@@ -368,8 +375,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     const int* __restrict__ in, int* __restrict__ out, int n, TileOut to, Levels L, LbView V,
     int nfix) {
     __shared__ TileSmem sm;
-    V.tag = V.ctl->epoch;  // set by the preceding kernel
+    if (V.tag == 0u) V.tag = V.ctl->epoch;  // a captured run: set by the preceding kernel
     const int b = (int)blockIdx.x;
+    if (V.GC_next && b == 0)  // no workgroup of this run uses that set
+        for (int i = (int)threadIdx.x; i < V.ngc; i += NT)
+            __hip_atomic_store(&V.GC_next[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (b < V.nt) {
         lb_solve<INCL>(in, out, n, b, sm, to, L, V);
         return;
@@ -380,14 +390,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
         lb_answer<INCL>(in, out, n, t, to, V, lds);
 }
 
-// next run tag (never 0, which fresh memory holds), error word and group
-// counters reset (one workgroup)
+// Before a captured run: the next captured-run tag (LB_TAG_CAPTURED_MIN and
+// up, wrapping within that range) and its group counters cleared (one
+// workgroup)
 __global__ __launch_bounds__(256) void k_lookback_begin(LbCtl* c, unsigned* gc, int ngc) {
     if (threadIdx.x == 0) {
         unsigned e = c->epoch + 1u;
-        if (e == 0u) e = 1u;
+        if (e < LB_TAG_CAPTURED_MIN) e = LB_TAG_CAPTURED_MIN;
         c->epoch = e;
-        c->err = 0;
     }
     for (int i = (int)threadIdx.x; i < ngc; i += (int)blockDim.x) gc[i] = 0u;
 }
@@ -414,17 +424,56 @@ static apsep_status run_lookback_t(apsep_scratch* s, const int* in, int* out, in
         V.G[l] = s->lb_G[l];
         V.gn[l] = l < nl ? sz[l] : 0;
     }
-    V.GC = s->lb_GC;
     int ngc = 0;  // group counters, level after level
     for (int l = 1; l < LB_LEVELS; l++) ngc += V.gn[l];
-    hipLaunchKernelGGL(k_lookback_begin, dim3(1), dim3(256), 0, st, s->lb_ctl, s->lb_GC, ngc);
+    V.ngc = ngc;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+        APSEP_CLEAR_HIP_ERROR();
+        return APSEP_ERR_UNSUPPORTED;
+    }
+    if (cs != hipStreamCaptureStatusNone && cs != hipStreamCaptureStatusActive)
+        return APSEP_ERR_UNSUPPORTED;
+    const bool cap = cs == hipStreamCaptureStatusActive;
+    if (cap) {
+        V.GC = s->lb_GC + LB_GC_CAPTURED * s->lb_gc_stride;
+        V.GC_next = nullptr;
+        V.tag = 0u;
+        hipLaunchKernelGGL(k_lookback_begin, dim3(1), dim3(256), 0, st, s->lb_ctl, V.GC, ngc);
+    } else {
+        unsigned tag = s->lb_tag + 1u;
+        if (tag >= LB_TAG_CAPTURED_MIN) {
+            // tag wrap (after ~2e9 eager runs): no published word may carry
+            // a tag of the new range
+            if (hipMemsetAsync(s->lb_M, 0, (size_t)s->lb_m_bytes, st) != hipSuccess ||
+                hipMemsetAsync(s->lb_CM, 0, (size_t)s->lb_cm_bytes, st) != hipSuccess) {
+                APSEP_CLEAR_HIP_ERROR();
+                return APSEP_ERR_LAUNCH;
+            }
+            for (int l = 1; l < LB_LEVELS; l++)
+                if (s->lb_G[l] &&
+                    hipMemsetAsync(s->lb_G[l], 0, (size_t)s->lb_g_bytes[l], st) != hipSuccess) {
+                    APSEP_CLEAR_HIP_ERROR();
+                    return APSEP_ERR_LAUNCH;
+                }
+            tag = 1u;
+        }
+        V.tag = tag;
+        V.GC = s->lb_GC + s->lb_parity * s->lb_gc_stride;
+        V.GC_next = s->lb_GC + (1 - s->lb_parity) * s->lb_gc_stride;
+    }
     // answer workgroups: one per block of NWAVE tiles, at most two per CU
     // (the kernel fits three per CU)
     const int nb = (nt + NWAVE - 1) / NWAVE;
     const int nfix = max(1, min(nb, 2 * s->num_cus));
     hipLaunchKernelGGL(k_lookback<INCL>, dim3(nt + nfix), dim3(NT), 0, st, in, out, n, to, L, V,
                        nfix);
-    return APSEP_LAUNCH_FAILED() ? APSEP_ERR_LAUNCH : APSEP_OK;
+    if (APSEP_LAUNCH_FAILED()) return APSEP_ERR_LAUNCH;
+    if (!cap) {
+        s->lb_tag = V.tag;
+        s->lb_parity ^= 1;
+    }
+    return APSEP_OK;
 }
 
 apsep_status run_lookback(apsep_scratch* s, const int32_t* in, int32_t* out, int n, int incl,

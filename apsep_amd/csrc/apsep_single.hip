@@ -321,7 +321,7 @@ int64_t single_scratch_bytes(int64_t n_cap, apsep_algo algo, int tile) {
         const LevelSizes ls = level_sizes_of(nt < 1 ? 1 : nt);
         b += (int64_t)sizeof(LbCtl);
         b += (round4(nt) + 4) * 8;                    // tile mins
-        b += (round4(nt) + 4) * 4;                    // group counters (fewer than nt)
+        b += (round4(nt) + 4) * 4 * LB_GC_SETS;       // group counter sets (fewer than nt each)
         b += (round4(ncm) + 4) * 8;                   // chunk mins
         for (int l = 1; l < ls.nl; l++) b += (round4(ls.sz[l]) + 4) * 8;  // group mins
         b += 256 * (4 + LB_LEVELS);
@@ -354,8 +354,12 @@ static bool carve_arrays(apsep_scratch* s) {
     if (s->algo == APSEP_ALGO_SPT) s->spt_link = (int*)carve(p, (round4(nt) + 4) * 4);
     if (s->algo == APSEP_ALGO_LOOKBACK) {
         s->lb_ctl = (LbCtl*)carve(p, (int64_t)sizeof(LbCtl));
+        s->lb_m_bytes = (round4(nt) + 4) * 8;
+        s->lb_cm_bytes = (round4(ncm) + 4) * 8;
+        for (int l = 1; l < nl; l++) s->lb_g_bytes[l] = (round4(sz[l]) + 4) * 8;
         s->lb_M = (u64*)carve(p, (round4(nt) + 4) * 8);
-        s->lb_GC = (unsigned*)carve(p, (round4(nt) + 4) * 4);
+        s->lb_GC = (unsigned*)carve(p, (round4(nt) + 4) * 4 * LB_GC_SETS);
+        s->lb_gc_stride = round4(nt) + 4;
         s->lb_CM = (u64*)carve(p, (round4(ncm) + 4) * 8);
         for (int l = 1; l < nl; l++) s->lb_G[l] = (u64*)carve(p, (round4(sz[l]) + 4) * 8);
     }
@@ -668,7 +672,15 @@ apsep_status single_error(apsep_scratch* s, int* code) {
             return APSEP_ERR_LAUNCH;
         }
         if (lerr) {
-            *code = 2;  // a look-back wait gave up; the next run resets the word
+            // a look-back wait gave up (the word holds that run's tag);
+            // cleared once read
+            const int zero = 0;
+            if (hipMemcpy(&s->lb_ctl->err, &zero, sizeof(int), hipMemcpyHostToDevice) !=
+                hipSuccess) {
+                APSEP_CLEAR_HIP_ERROR();
+                return APSEP_ERR_LAUNCH;
+            }
+            *code = 2;
             return APSEP_OK;
         }
     }

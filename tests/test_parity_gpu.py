@@ -308,9 +308,10 @@ def test_time_kernels_leaves_valid_output(torch_mod, algo):
 
 
 def test_lookback_graph_replay(torch_mod):
-    """The look-back sequence keeps no host-baked state: capture it once
-    with torch.cuda.graph, then replay with new keys in the same buffer,
-    eager work and synchronisation between replays; every replay checked."""
+    """The captured look-back sequence keeps no host-baked state: capture it
+    once with torch.cuda.graph, then replay with new keys in the same
+    buffer, eager solves on the same scratch and synchronisation between
+    replays; every replay and every eager solve checked."""
     import apsep_amd as A
     torch = torch_mod
     n = (1 << 21) + 5
@@ -337,3 +338,11 @@ def test_lookback_graph_replay(torch_mod):
             torch.cuda.synchronize()
             s.check()
             check(out.cpu().numpy(), a, False, "lookback replay %d %s" % (rep, kind))
+            # eager runs on the same scratch between replays (host-kept tags
+            # and counter sets beside the replay's device-kept ones)
+            for e in range(2):
+                out.fill_(int(POISON))
+                s.run(x, out)
+                torch.cuda.synchronize()
+                s.check()
+                check(out.cpu().numpy(), a, False, "lookback eager %d after replay %d" % (e, rep))
