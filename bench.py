@@ -257,7 +257,8 @@ def per_array_baselines(torch, apsep_amd, stream, xp, outp, pstart, lengths, lau
 def run_batch(args, torch, apsep_amd, cpu: bool):
     """Segmented batch: 8192 segments with log-uniform lengths in [64, 2^20]
     (about 8.7e8 keys), one class per segment.  The batched solve (the
-    autotuner's tile shape, eager and hipGraph replay) against one solve per
+    autotuner's tile shape and launch sequence, eager and hipGraph replay;
+    one persistent launch per batch timed beside it) against one solve per
     array on the same arrays (apsep_run on a persistent scratch, and
     apsep_launch)."""
     import numpy as np
@@ -272,7 +273,14 @@ def run_batch(args, torch, apsep_amd, cpu: bool):
         tuned = p.tune(x, offs, out, total=total, stream=stream)
         res["tuned_mode"] = "graph" if tuned[1] < tuned[0] else "eager"
         res["tile_lanes"] = p.tile_lanes
+        res["tuned_persistent"] = p.tuned_persistent(int(total - 1).bit_length())
         p.use_graph(False)
+        # the same batch as one persistent launch (tuned tile shape)
+        p.set_persistent(True)
+        run()
+        res["persistent_ms"] = _timed(torch, stream, run, max(3, args.batch_steps // 2))
+        p.check()
+        p.set_persistent(res["tuned_persistent"])
         for _ in range(args.batch_warmup):
             run()
         res["eager_ms"] = _timed(torch, stream, run, args.batch_steps)
@@ -294,6 +302,8 @@ def run_batch(args, torch, apsep_amd, cpu: bool):
         "ms_per_batch": round(best, 4),
         "eager_ms": round(res["eager_ms"], 4), "graph_ms": round(res["graph_ms"], 4),
         "autotuned_mode": res["tuned_mode"], "autotuned_tile_lanes": res["tile_lanes"],
+        "autotuned_persistent": bool(res["tuned_persistent"]),
+        "persistent_launch_ms": round(res["persistent_ms"], 4),
         "per_array_run_ms": round(run_ms, 3),
         "per_array_launch_ms_est": round(launch_ms * nseg, 2),
         "per_array_launch_sampled": min(nseg, nseg if args.workload == "batch" else 1024),
