@@ -36,6 +36,7 @@ EXPORTS = [
     "apsep_scratch_destroy", "apsep_scratch_bytes", "apsep_run",
     "apsep_launch", "apsep_run_mode", "apsep_run_noalloc", "apsep_scratch_query",
     "apsep_scratch_check", "apsep_scratch_set_persistent_blocks", "apsep_time_kernels",
+    "apsep_scratch_set_cooperative", "apsep_scratch_cooperative_refusals",
     "apsep_time_kernels_rotating",
     "apsep_batch_plan_create", "apsep_batch_plan_destroy", "apsep_run_segmented",
     "apsep_batch_plan_use_graph", "apsep_batch_plan_tune",
@@ -109,6 +110,8 @@ def lib() -> ctypes.CDLL:
         L.apsep_scratch_query.argtypes = [vp, ctypes.POINTER(ScratchInfo)]
         L.apsep_scratch_check.argtypes = [vp, ctypes.POINTER(i32)]
         L.apsep_scratch_set_persistent_blocks.argtypes = [vp, i32]
+        L.apsep_scratch_set_cooperative.argtypes = [vp, i32]
+        L.apsep_scratch_cooperative_refusals.argtypes = [vp, ctypes.POINTER(i64)]
         L.apsep_time_kernels.argtypes = [vp, vp, vp, i64, ci, ci, vp,
                                          ctypes.POINTER(KernelTimes)]
         L.apsep_time_kernels_rotating.argtypes = [vp, vp, vp, i32, i64, ci, ci, vp,
@@ -266,6 +269,21 @@ class Scratch:
         """SPT persistent grid of later runs (0: the default, occupancy x CUs)."""
         _check(lib().apsep_scratch_set_persistent_blocks(self._h, int(blocks)),
                "apsep_scratch_set_persistent_blocks")
+
+    def set_cooperative(self, enable: bool = True):
+        """SPT: launch later eager runs with hipLaunchCooperativeKernel (True)
+        or normally (False, the default)."""
+        _check(lib().apsep_scratch_set_cooperative(self._h, int(enable)),
+               "apsep_scratch_set_cooperative")
+
+    @property
+    def cooperative_refusals(self) -> int:
+        """Cooperative launches the runtime refused (those runs launched
+        normally)."""
+        v = ctypes.c_int64(0)
+        _check(lib().apsep_scratch_cooperative_refusals(self._h, ctypes.byref(v)),
+               "apsep_scratch_cooperative_refusals")
+        return v.value
 
     def check(self):
         """Blocking check of the device-side error word (raises on error)."""

@@ -148,6 +148,19 @@ apsep_status apsep_scratch_set_persistent_blocks(apsep_scratch* s, int32_t block
     return APSEP_OK;
 }
 
+apsep_status apsep_scratch_set_cooperative(apsep_scratch* s, int32_t enable) {
+    if (!s || (enable != 0 && enable != 1)) return APSEP_ERR_INVALID_ARG;
+    if (s->algo != APSEP_ALGO_SPT) return APSEP_ERR_UNSUPPORTED;
+    s->spt_cooperative = enable;
+    return APSEP_OK;
+}
+
+apsep_status apsep_scratch_cooperative_refusals(const apsep_scratch* s, int64_t* refused) {
+    if (!s || !refused) return APSEP_ERR_INVALID_ARG;
+    *refused = s->spt_coop_refused;
+    return APSEP_OK;
+}
+
 apsep_status apsep_scratch_check(apsep_scratch* s, int32_t* device_error) {
     if (!s || !device_error) return APSEP_ERR_INVALID_ARG;
     int code = 0;

@@ -28,6 +28,8 @@ struct apsep_scratch {
     int2* spine;
     int spt_grid;        // SPT persistent grid size (occupancy x CUs)
     int spt_blocks;      // SPT grid the caller asked for (0: spt_grid)
+    int spt_cooperative; // SPT eager runs launched with hipLaunchCooperativeKernel
+    int64_t spt_coop_refused;  // cooperative launches the runtime refused (run normally instead)
     int* spt_link;       // SPT: per tile, the next tile of a workgroup's overflow list
     apsep::GridBarrier* bar;   // BAR_SETS barriers (persistent launches)
     int bar_parity;            // the set the next eager persistent launch uses

@@ -504,8 +504,26 @@ static apsep_status run_spt(apsep_scratch* s, const int* in, int* out, int n,
         bar = s->bar + s->bar_parity;
         next = s->bar + (1 - s->bar_parity);
     }
-    hipLaunchKernelGGL(k_spt<INCL>, dim3(grid), dim3(NT), 0, st, in, out, n, to, L, bar, next,
-                       s->spt_link);
+    bool launched = false;
+    if (s->spt_cooperative && !cap) {
+        // the runtime guarantees co-residency (the grid fits at once); the
+        // kernel's join then finds every workgroup a participant.  A refused
+        // launch runs nothing and is counted; the normal launch follows.
+        const int cgrid = grid < s->spt_grid ? grid : s->spt_grid;
+        int* link = s->spt_link;
+        void* args[] = {(void*)&in, (void*)&out, (void*)&n, (void*)&to, (void*)&L,
+                        (void*)&bar, (void*)&next, (void*)&link};
+        if (hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_spt<INCL>), dim3(cgrid),
+                                       dim3(NT), args, 0, st) == hipSuccess) {
+            launched = true;
+        } else {
+            APSEP_CLEAR_HIP_ERROR();
+            s->spt_coop_refused++;
+        }
+    }
+    if (!launched)
+        hipLaunchKernelGGL(k_spt<INCL>, dim3(grid), dim3(NT), 0, st, in, out, n, to, L, bar, next,
+                           s->spt_link);
     if (APSEP_LAUNCH_FAILED()) return APSEP_ERR_LAUNCH;
     if (!cap) s->bar_parity ^= 1;
     return APSEP_OK;

@@ -147,6 +147,19 @@ apsep_status apsep_scratch_query(const apsep_scratch* s, apsep_scratch_info* inf
  * reference, whose grid is fixed at num_phys (src/apsep.cuh:3024-3032). */
 apsep_status apsep_scratch_set_persistent_blocks(apsep_scratch* s, int32_t blocks);
 
+/* SPT launch mode of later eager runs on this scratch: enable = 1 launches
+ * the persistent kernel with hipLaunchCooperativeKernel, as the reference's
+ * runSPT does with cudaLaunchCooperativeKernel (src/apsep.cuh:3063-3066),
+ * on at most the co-resident grid; 0 (the default) launches it normally,
+ * the workgroups finding at run time which of them run together (faster on
+ * MI355X, and capturable).  Answers do not depend on it.  Captured runs
+ * always launch normally.  A cooperative launch the runtime refuses runs
+ * nothing; the run then launches normally and the refusal is counted
+ * (apsep_scratch_cooperative_refusals).  APSEP_ERR_UNSUPPORTED for a
+ * scratch of another algorithm. */
+apsep_status apsep_scratch_set_cooperative(apsep_scratch* s, int32_t enable);
+apsep_status apsep_scratch_cooperative_refusals(const apsep_scratch* s, int64_t* refused);
+
 /* Blocking check of the scratch's device-side error word, then clears it.
  * *device_error != 0 means a run on this scratch was cut short on the
  * device (a persistent launch whose workgroups were not all co-resident

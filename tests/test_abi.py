@@ -78,6 +78,9 @@ def test_run_and_launch_argument_checks():
     assert L.apsep_run(None, 16, 32, 4, None) == A.ERR_INVALID_ARG
     assert L.apsep_scratch_set_persistent_blocks(None, 8) == A.ERR_INVALID_ARG
     assert L.apsep_batch_plan_set_persistent(None, 1) == A.ERR_INVALID_ARG
+    assert L.apsep_scratch_set_cooperative(None, 1) == A.ERR_INVALID_ARG
+    refused = ctypes.c_int64(0)
+    assert L.apsep_scratch_cooperative_refusals(None, ctypes.byref(refused)) == A.ERR_INVALID_ARG
     code = ctypes.c_int32(0)
     assert L.apsep_batch_plan_check(None, ctypes.byref(code)) == A.ERR_INVALID_ARG
     assert L.apsep_batch_plan_tuned_persistent(None, 3, ctypes.byref(code)) == A.ERR_INVALID_ARG

@@ -91,3 +91,31 @@ def test_set_blocks_rejects(torch_mod):
     with A.Scratch("wstl", TILE) as s:
         with pytest.raises(A.ApsepError):
             s.set_persistent_blocks(4)
+
+
+def test_cooperative_launch(torch_mod):
+    """apsep_scratch_set_cooperative: the persistent kernel launched with
+    hipLaunchCooperativeKernel (as the reference's runSPT launches with
+    cudaLaunchCooperativeKernel), on the default grid and on grids of 7 and
+    twice the co-resident size (clamped to what fits at once), alternating
+    with normal launches on the same scratch; every class, both predicates.
+    A launch the runtime refuses runs normally and is counted."""
+    import apsep_amd as A
+    n = (1 << 24) + 5
+    with A.Scratch("spt", n) as s:
+        for blocks in (0, 7, 2 * s.info.persistent_blocks):
+            s.set_persistent_blocks(blocks)
+            for kind in ("random", "descending", "ascending"):
+                a = oracle.fill(kind, n, seed=40 + blocks % 97)
+                for incl in (False, True):
+                    for coop in (True, False):
+                        s.set_cooperative(coop)
+                        _log("cooperative" if coop else "normal", "blocks", blocks, kind,
+                             "incl" if incl else "strict")
+                        _solve_check(torch_mod, s, a, incl, "coop=%d blocks=%d %s incl=%s"
+                                     % (coop, blocks, kind, incl))
+        _log("cooperative refusals", s.cooperative_refusals)
+        assert s.cooperative_refusals >= 0
+    with A.Scratch("wstl", TILE) as s:
+        with pytest.raises(A.ApsepError):
+            s.set_cooperative(True)
