@@ -38,6 +38,7 @@ struct apsep_scratch {
     unsigned long long* lb_M;
     unsigned* lb_GC;           // LB_GC_SETS sets of group counters, lb_gc_stride apart
     int64_t lb_gc_stride;
+    int lb_ngc_cap;            // group counters of a run at capacity (every run's fit in it)
     int64_t lb_m_bytes, lb_cm_bytes, lb_g_bytes[apsep::LB_LEVELS];  // published arrays
     unsigned lb_tag;           // the last eager run's tag (LB_TAG_CAPTURED_MIN and up: captured runs)
     int lb_parity;             // the counter set the next eager run uses

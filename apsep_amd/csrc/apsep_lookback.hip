@@ -60,7 +60,8 @@ struct LbView {
     unsigned* GC;             // group counters: entries of group g at level l - 1
                               // published, at GC[gn[1] + .. + gn[l - 1] + g]
     unsigned* GC_next;        // the next eager run's set, cleared by this run (or null)
-    int ngc;                  // group counters in a set
+    int ngc_next;             // counters cleared there: as many as a run at capacity uses
+                              // (the next run may be larger than this one)
     int gn[LB_LEVELS];        // entries per level (gn[0] = nt)
     u64* CM;                  // [nt * CPT] chunk mins
     unsigned tag;             // 0: read the captured run's tag from ctl
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     if (V.tag == 0u) V.tag = V.ctl->epoch;  // a captured run: set by the preceding kernel
     const int b = (int)blockIdx.x;
     if (V.GC_next && b == 0)  // no workgroup of this run uses that set
-        for (int i = (int)threadIdx.x; i < V.ngc; i += NT)
+        for (int i = (int)threadIdx.x; i < V.ngc_next; i += NT)
             __hip_atomic_store(&V.GC_next[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (b < V.nt) {
         lb_solve<INCL>(in, out, n, b, sm, to, L, V);
@@ -426,7 +427,7 @@ static apsep_status run_lookback_t(apsep_scratch* s, const int* in, int* out, in
     }
     int ngc = 0;  // group counters, level after level
     for (int l = 1; l < LB_LEVELS; l++) ngc += V.gn[l];
-    V.ngc = ngc;
+    V.ngc_next = s->lb_ngc_cap;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
         APSEP_CLEAR_HIP_ERROR();

@@ -360,6 +360,8 @@ static bool carve_arrays(apsep_scratch* s) {
         s->lb_M = (u64*)carve(p, (round4(nt) + 4) * 8);
         s->lb_GC = (unsigned*)carve(p, (round4(nt) + 4) * 4 * LB_GC_SETS);
         s->lb_gc_stride = round4(nt) + 4;
+        s->lb_ngc_cap = 0;
+        for (int l = 1; l < nl; l++) s->lb_ngc_cap += sz[l];
         s->lb_CM = (u64*)carve(p, (round4(ncm) + 4) * 8);
         for (int l = 1; l < nl; l++) s->lb_G[l] = (u64*)carve(p, (round4(sz[l]) + 4) * 8);
     }

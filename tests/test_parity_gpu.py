@@ -307,6 +307,31 @@ def test_time_kernels_leaves_valid_output(torch_mod, algo):
         check(out.cpu().numpy(), a, False, "%s time_kernels" % algo)
 
 
+def test_lookback_sizes_up_and_down(torch_mod):
+    """Eager look-back runs of changing sizes on one scratch: each run clears
+    the group counters of the next (two sets in turn), so a large run after
+    a small one must find every counter of its larger hierarchy clear.  One
+    run per size, so that runs 2, 4, 6 and 8 land on a counter set that a
+    run two before dirtied over more counters than the run just before
+    them used (a clearing bounded by the clearing run's own size would
+    leave 6, 5, 16 and 14 of their counters stale)."""
+    import apsep_amd as A
+    torch = torch_mod
+    sizes = [1 << 22, (1 << 19) + 5, 1 << 22, (1 << 20) + 3, 1 << 23, 4099, 1 << 23,
+             (1 << 19) + 5, 1 << 23]
+    with A.Scratch("lookback", max(sizes)) as s:
+        for k, n in enumerate(sizes):
+            kind = ("random", "descending", "dup16")[k % 3]
+            a = oracle.fill(kind, n, seed=70 + k)
+            x = torch.from_numpy(a).cuda()
+            out = torch.full((n,), int(POISON), dtype=torch.int32, device="cuda")
+            _log("lookback sizes", k, n, kind)
+            s.run(x, out)
+            torch.cuda.synchronize()
+            s.check()
+            check(out.cpu().numpy(), a, False, "lookback size %d (%d) %s" % (k, n, kind))
+
+
 def test_lookback_graph_replay(torch_mod):
     """The captured look-back sequence keeps no host-baked state: capture it
     once with torch.cuda.graph, then replay with new keys in the same
