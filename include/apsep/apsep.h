@@ -161,9 +161,10 @@ apsep_status apsep_scratch_set_cooperative(apsep_scratch* s, int32_t enable);
 apsep_status apsep_scratch_cooperative_refusals(const apsep_scratch* s, int64_t* refused);
 
 /* Blocking check of the scratch's device-side error word, then clears it.
- * *device_error != 0 means a run on this scratch was cut short on the
- * device (a persistent launch whose workgroups were not all co-resident
- * times out at its grid barrier instead of hanging); its output is invalid.
+ * *device_error != 0 means a run on this scratch since the last check was
+ * cut short on the device (1: a persistent launch timed out at its grid
+ * barrier instead of hanging; 2: a look-back wait gave up); its output is
+ * invalid.  Later runs are not affected by an earlier run's error.
  * No counterpart in the reference, which has no device-side error path. */
 apsep_status apsep_scratch_check(apsep_scratch* s, int32_t* device_error);
 
